@@ -1,0 +1,175 @@
+"""Benchmark: cell-RK-stage updates/s of the explicit finite-volume step on
+one MI355X, mixed precision vs the FP64 path, with the HBM roofline of the
+dominant kernel and a CPU baseline (OpenMP FP64 oracle on a bounded sample).
+
+A step = SSP-RK3 (3 stages: pass G + pass F each, dt from the CFL condition
+at stage 0) + the per-cell phase relaxation, on the 48M-tet shock-bubble
+workload (200^3 jittered hexes x 6 tets, 64 vapour bubbles, WALE on).
+Inputs are resident in HBM before the timed region.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 200]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK = 8.0e12  # B/s, MI355X
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def stage_bytes(neq, s, fc, s_geo, s_grad):
+    """Algorithmic bytes per cell-RK-stage, split by pass (compulsory
+    traffic of the two-pass design, every array touched once per pass):
+      pass G: U read, geometry, face states (4 NR values) written
+      pass F: U read, U0 read (stages 2..s), U_new written, geometry,
+              face states read.
+    Geometry per pass: per face fc2el(8) + bc(4) + n.A (3 S) + x_f (3 S),
+    per cell V + x_c (4 S) + 4 slot ids (16)."""
+    geo = fc * (12 + 6 * s_geo) + (4 * s_geo + 16)
+    fs = 4 * neq * s_grad
+    g = 8 * neq + geo + fs
+    f_nou0 = 8 * neq + 8 * neq + geo + fs
+    f_u0 = f_nou0 + 8 * neq
+    total = g + 8 * neq * (2 + (s - 1) / s) + geo + fs  # pass G + pass F averaged over the s stages
+    return dict(g=g, f_nou0=f_nou0, f_u0=f_u0, total=total, geo=geo)
+
+
+def relax_bytes(neq):
+    return 8 * neq + 48
+
+
+def run_gpu(case, precision, steps, warmup, device=0):
+    from fooskagpu_amd import GpuSolver
+    p = dict(case.params, precision=precision)
+    g = GpuSolver(p, case.mesh, case.U, device=device)
+    g.sync()
+    if warmup:
+        g.run_steps(warmup)
+    g.upload_state(case.U)
+    g.sync()
+    g.set_timing(True)
+    t0 = time.perf_counter()
+    dt, tphys = g.run_steps(steps)  # returns after a stream sync
+    el = time.perf_counter() - t0
+    kt = g.kernel_times()
+    U = g.download_state()
+    g.close()
+    return el, kt, U, dt
+
+
+def cpu_baseline(target_s=15.0):
+    """FP64 oracle with OpenMP on a bounded sample of the same workload
+    (shock-bubble at 40^3 hexes x 6 = 384k cells), steps until ~target_s."""
+    from fooskagpu_amd import cases
+    from oracle.pyoracle import Oracle
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    c = cases.shock_bubble(n=40, nbub=64)
+    o = Oracle(c.params, c.mesh, c.U, omp=True)
+    steps, el = 0, 0.0
+    while el < target_s and steps < 50:
+        t0 = time.perf_counter()
+        o.run_steps(1)
+        el += time.perf_counter() - t0
+        steps += 1
+    v = c.mesh.n_cells * 3 * steps / el
+    return dict(value=v, unit="cell-RK-stage updates/s", cores=threads, kind="port",
+                sample=f"FP64 OpenMP oracle, shock-bubble 40^3x6 = {c.mesh.n_cells} cells, {steps} SSP-RK3 steps "
+                       f"incl. relaxation, {el:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=200, help="hexes per axis (200 -> 48M tets)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (halo exchange is out of scope); "
+              "use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from fooskagpu_amd import FP64, MIXED, cases
+    t0 = time.perf_counter()
+    case = cases.shock_bubble(n=args.n, nbub=64)
+    C = case.mesh.n_cells
+    fc = case.mesh.n_faces / C
+    log(f"mesh {C} cells, {case.mesh.n_faces} faces (F/C {fc:.4f}) built in {time.perf_counter() - t0:.1f} s")
+    s = 3
+    res = {}
+    for name, prec in (("mixed", MIXED), ("fp64", FP64)):
+        if name == "fp64" and args.no_fp64:
+            continue
+        log(f"{name}: warmup {args.warmup}, timing {args.steps} steps")
+        el, kt, U, dt = run_gpu(case, prec, args.steps, args.warmup)
+        rate = C * s * args.steps / el
+        log(f"{name}: {el:.3f} s, {rate:.4g} cell-stage/s, last dt {dt:.3e}, finite={bool(np.isfinite(U).all())}")
+        for k, (n, ms) in sorted(kt.items(), key=lambda kv: -kv[1][1]):
+            log(f"   {k:14s} {n:5d} launches {ms:9.2f} ms  avg {ms / max(n, 1):8.3f} ms")
+        res[name] = dict(el=el, rate=rate, kt=kt, U=U)
+
+    mixed = res["mixed"]
+    sb = 4
+    B = stage_bytes(10, s, fc, sb, sb)
+    kt = mixed["kt"]
+    n_f, ms_f = kt.get("pass_f", (0, 0.0))
+    avg_f = ms_f / max(n_f, 1) / 1e3
+    achieved = B["f_u0"] * C / avg_f if avg_f > 0 else 0.0
+    whole = (B["total"] * C * s * args.steps + relax_bytes(10) * C * args.steps) / mixed["el"]
+    out = {
+        "metric": "cell-RK-stage updates/s, 1 MI355X: mixed+surrogate vs FP64; % of HBM peak",
+        "value": mixed["rate"],
+        "unit": "cell-RK-stage updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": mixed["el"] / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32+fp64",
+        "data": "synthetic",
+        "config": {"workload": f"shock-bubble {args.n}^3 jittered hexes x 6 tets ({C} cells), 64 vapour bubbles, "
+                               f"WALE, SSP-RK3, CFL 0.1, plain Newton relaxation, mixed precision",
+                   "cells": C, "faces": case.mesh.n_faces},
+        "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "kernel": "pass_f", "bytes_per_cell": B["f_u0"]},
+        "whole_step_hbm_frac": whole / HBM_PEAK,
+    }
+    if "fp64" in res:
+        f64 = res["fp64"]
+        Bd = stage_bytes(10, s, fc, 8, 8)
+        out["fp64"] = {"value": f64["rate"], "ms_per_step": f64["el"] / args.steps * 1e3,
+                       "whole_step_hbm_frac": (Bd["total"] * C * s * args.steps + relax_bytes(10) * C * args.steps)
+                       / f64["el"] / HBM_PEAK}
+        out["speedup_mixed_vs_fp64"] = mixed["rate"] / f64["rate"]
+        Uo, Um = f64["U"], mixed["U"]
+        rho_o, rho_m = Uo[:, 1] + Uo[:, 2], Um[:, 1] + Um[:, 2]
+        out["mixed_vs_fp64_gpu_rel_l1"] = {
+            "density": float(np.abs(rho_m - rho_o).sum() / np.abs(rho_o).sum()),
+            "momentum": float(np.abs(Um[:, 3:6] - Uo[:, 3:6]).sum() / max(np.abs(Uo[:, 3:6]).sum(), 1e-300)),
+            "energy": float(np.abs(Um[:, 8] - Uo[:, 8]).sum() / np.abs(Uo[:, 8]).sum()),
+            "vapour_fraction": float(np.abs(Um[:, 0] - Uo[:, 0]).sum() / np.abs(Uo[:, 0]).sum()),
+        }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        out["cpu_baseline"] = cpu_baseline()
+        log(f"cpu baseline {out['cpu_baseline']['value']:.4g} cell-stage/s")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

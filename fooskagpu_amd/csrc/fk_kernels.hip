@@ -1,0 +1,1462 @@
+// MI355X (gfx950) kernels of the explicit unstructured finite-volume step
+// and the C ABI of include/fooska.h.
+//
+// Layout in HBM (one context):
+//   U        3 rotating buffers [nElem][NEQ] double (reference AoS consVec)
+//   slots    4 per cell, ordered by ascending face id:
+//              nbr[4C]   neighbour cell, or -bc on boundary faces
+//              mir[4C]   (mirror slot in the neighbour << 1) | owner flag
+//              geo[4C]   (n.x, n.y, n.z, A) of the face (owner normal)
+//              rvec[4C]  (x_f - x_c) of the face seen from this cell
+//              wls[4C]   WALE filter width 0.325 ((V_o+V_n)/2)^0.3333333333333
+//   face     [4C][NEQ] T: MUSCL face state of every cell face slot
+//   gvel     [C][NG]  T: velocity gradient (+ temperature gradient)
+//   V, Vcfl  [C] double: volume and V^0.3333
+// T = double (FP64 path) or float (mixed path).
+//
+// Pass G (fk_compute_face_values) = computePrimVec + computeGrad +
+// computeFaceValues; pass F (fk_compute_rhs[_update]) = computeRHS (+ viscous)
+// + update.  Both are cell-centric gathers: each thread owns one cell and
+// sums its four faces in slot order, so no atomics and a fixed summation
+// order.  The Riemann problem of an interior face is solved by both of its
+// cells (identically), which trades ALU for not writing/reading per-face
+// fluxes through HBM.
+
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/fooska.h"
+#include "fk_device.h"
+
+namespace {
+
+thread_local std::string g_err;
+int fail(int code, const std::string &m) {
+  g_err = m;
+  return code;
+}
+#define HIPCHK(x)                                                                              \
+  do {                                                                                         \
+    hipError_t e_ = (x);                                                                       \
+    if (e_ != hipSuccess) return fail(FK_EDEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
+  } while (0)
+
+constexpr int kBlock = 256;
+
+template <typename T>
+struct V4 {
+  T x, y, z, w;
+};
+
+struct Timer {
+  std::vector<std::string> names;
+  std::vector<int64_t> counts;
+  std::vector<double> ms;
+  struct Pending {
+    int id;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending;
+  std::vector<hipEvent_t> pool;
+};
+
+}  // namespace
+
+struct fk_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  fk_params p{};
+  int neq = 5;
+  int64_t nc = 0, nf = 0;
+  bool have_mesh = false, have_state = false, debug = false, timing = false;
+  // mesh
+  int32_t *nbr = nullptr, *mir = nullptr;
+  void *geo = nullptr, *rvec = nullptr, *wls = nullptr;  // T-typed, sized for double
+  double *vol = nullptr, *volcfl = nullptr;
+  std::vector<int32_t> slot_face;  // host copy of slot -> face id
+  // fields
+  double *U[3] = {nullptr, nullptr, nullptr};
+  int ia = 0, iu0 = 0;
+  void *face = nullptr, *gvel = nullptr;
+  double *rhs = nullptr, *dtc = nullptr, *red = nullptr, *dscal = nullptr;
+  double *dgrad = nullptr, *dtheta = nullptr;
+  int *icount = nullptr;
+  long long *lcount = nullptr;
+  double dt = 0, time = 0;
+  int64_t step = 0;
+  bool rhs_valid = false;
+  Timer tm;
+  // host geometry kept to rebuild the precision-dependent arrays
+  std::vector<double> h_geo, h_r, h_ls;
+};
+
+namespace {
+
+// ------------------------------------------------------------------------
+// kernels
+// ------------------------------------------------------------------------
+
+template <int NEQ>
+struct Model {
+  static constexpr int iu = NEQ == 5 ? 1 : 3;  // first velocity slot of W
+};
+
+// number of gradient values kept for the viscous pass
+template <int NEQ, bool HEAT>
+constexpr int ngrad() {
+  return 9 + (HEAT ? 3 : 0);
+}
+
+template <typename T>
+__device__ __forceinline__ V4<T> ld4(const void *p, int64_t i) {
+  return reinterpret_cast<const V4<T> *>(p)[i];
+}
+
+// Pass G: first-order states, Green-Gauss gradient, neighbour min/max,
+// limiter and MUSCL face states (computePrimVec.cu:38-80,
+// computeGrad.cu:51-91, computeFaceValues.cu:29-113).
+template <int NEQ, typename T, bool VISC, bool HEAT>
+__global__ void __launch_bounds__(kBlock) pass_g(fk_params P, int64_t nc, const double *__restrict__ U,
+                                                 const int32_t *__restrict__ nbr, const int32_t *__restrict__ mir,
+                                                 const void *__restrict__ geo, const void *__restrict__ rvec,
+                                                 const double *__restrict__ vol, T *__restrict__ face,
+                                                 T *__restrict__ gvel, double *__restrict__ dgrad,
+                                                 double *__restrict__ dtheta) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= nc) return;
+  const fk::Mat<double> md = fk::make_mat<double>(P);
+  const fk::Mat<T> mt = fk::make_mat<T>(P);
+  double Wd[NEQ];
+  fk::cons2prim<NEQ>(md, U + c * NEQ, Wd);
+  T W[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) W[v] = (T)Wd[v];
+  T g[NEQ][3], mn[NEQ], mx[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) {
+    g[v][0] = g[v][1] = g[v][2] = 0;
+    mn[v] = (T)1e12;
+    mx[v] = (T)-1e12;
+  }
+  const T V = (T)vol[c];
+  for (int j = 0; j < 4; ++j) {
+    const int64_t s = c * 4 + j;
+    const int nb = nbr[s];
+    const bool own = mir[s] & 1;
+    const V4<T> ga = ld4<T>(geo, s);
+    const T n[3] = {ga.x, ga.y, ga.z};
+    T Wn[NEQ];
+    if (nb >= 0) {
+      double t[NEQ];
+      fk::cons2prim<NEQ>(md, U + (int64_t)nb * NEQ, t);
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) Wn[v] = (T)t[v];
+    } else {
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) Wn[v] = W[v];
+      fk::ghost<NEQ, T>(mt, -nb, n, Wn);
+    }
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) {
+      const T o = Wn[v];
+      if (!(mx[v] >= o)) mx[v] = o;
+      if (!(mn[v] <= o)) mn[v] = o;
+      const T avg = (W[v] + Wn[v]) / 2;
+      if (own) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) g[v][d] += avg * n[d] * ga.w / V;
+      } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) g[v][d] += -avg * n[d] * ga.w / V;
+      }
+    }
+  }
+  // limiter: theta = min over faces of clamp(phi, 0, 1)
+  T th[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) th[v] = (T)1e9;
+  V4<T> rr[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) rr[j] = ld4<T>(rvec, c * 4 + j);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) {
+      const T wu = W[v] + rr[j].x * g[v][0] + rr[j].y * g[v][1] + rr[j].z * g[v][2];
+      T phi;
+      if (wu - W[v] != 0)
+        phi = (wu > W[v]) ? (mx[v] - W[v]) / (2 * (wu - W[v])) : (mn[v] - W[v]) / (2 * (wu - W[v]));
+      else
+        phi = 1;
+      const T tt = fmax((T)0, fmin(phi, (T)1));
+      if (!(th[v] <= tt)) th[v] = tt;
+    }
+  }
+  // MUSCL face states
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    T *out = face + (c * 4 + j) * NEQ;
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) {
+      const T ext = rr[j].x * g[v][0] + rr[j].y * g[v][1] + rr[j].z * g[v][2];
+      out[v] = W[v] + th[v] * ext;
+    }
+  }
+  if constexpr (VISC) {
+    constexpr int iu = Model<NEQ>::iu;
+    constexpr int NG = ngrad<NEQ, HEAT>();
+    T *G = gvel + c * NG;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) G[i * 3 + d] = g[iu + i][d];
+    if constexpr (HEAT) {
+      const T gm = (T)P.gamma[0], pi = (T)P.pInf[0];
+      const T R = (T)P.Cp[0] * (gm - 1) / gm;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) G[9 + d] = (g[4][d] - ((W[4] + pi) / W[0]) * g[0][d]) / (W[0] * R);
+    }
+  }
+  if (dgrad) {
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) {
+      dtheta[c * NEQ + v] = (double)th[v];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) dgrad[(c * NEQ + v) * 3 + d] = (double)g[v][d];
+    }
+  }
+}
+
+// Pass F: Riemann problem on each face of the cell, non-conservative terms,
+// pressure correction, viscous/WALE terms, per-cell gather and (MODE 0) the
+// RK stage, or (MODE 1) residual + signal speed for the time step.
+// (computeRHS.cu:78-146, computeViscousStresses.cu:66-174, update.cu:3-103,
+// computeTimeStep.cu:97-159)
+template <int NEQ, typename T, bool VISC, bool HEAT, int MODE>
+__global__ void __launch_bounds__(kBlock) pass_f(fk_params P, int64_t nc, const double *__restrict__ U,
+                                                 const double *__restrict__ U0, double *__restrict__ Unew,
+                                                 const int32_t *__restrict__ nbr, const int32_t *__restrict__ mir,
+                                                 const void *__restrict__ geo, const T *__restrict__ wls,
+                                                 const double *__restrict__ vol, const double *__restrict__ volcfl,
+                                                 const T *__restrict__ face, const T *__restrict__ gvel,
+                                                 double *__restrict__ rhs_out, double *__restrict__ dtc, int irk,
+                                                 double dt_val, const double *__restrict__ dt_ptr) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= nc) return;
+  const fk::Mat<double> md = fk::make_mat<double>(P);
+  const fk::Mat<T> mt = fk::make_mat<T>(P);
+  constexpr int iu = Model<NEQ>::iu;
+  constexpr int NG = ngrad<NEQ, HEAT>();
+  const double *Uc = U + c * NEQ;
+  // side terms of this cell (fluxSource / computeCenterPress, two-phase)
+  T avf = 0, pLiq = 0, pGas = 0;
+  double pCd = 0;
+  if constexpr (NEQ == 10) {
+    const double a = fk::clamp_avf(md, Uc[0]);
+    const double rl = Uc[1] / a, rg = Uc[2] / (1 - a);
+    const double el = Uc[6] / (a * rl), eg = Uc[7] / ((1 - a) * rg);
+    const double pl = fk::p_of(md, 0, rl, el), pg = fk::p_of(md, 1, rg, eg);
+    pCd = a * pl + (1 - a) * pg;
+    avf = (T)a;
+    pLiq = (T)pl;
+    pGas = (T)pg;
+  }
+  double rhs[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) rhs[v] = 0.0;
+  T sig = (T)-1e9;
+  T Wc1[NEQ];  // first-order state of this cell (viscous boundary ghost)
+  if constexpr (VISC) {
+    double t[NEQ];
+    fk::cons2prim<NEQ>(md, Uc, t);
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) Wc1[v] = (T)t[v];
+  }
+  for (int j = 0; j < 4; ++j) {
+    const int64_t s = c * 4 + j;
+    const int nb = nbr[s];
+    const int mv = mir[s];
+    const bool own = mv & 1;
+    const V4<T> ga = ld4<T>(geo, s);
+    const T n[3] = {ga.x, ga.y, ga.z};
+    const T A = ga.w;
+    T L[NEQ], R[NEQ];
+    const T *Fs = face + s * NEQ;
+    if (nb >= 0) {
+      const T *Fn = face + (int64_t)(mv >> 1) * NEQ;
+      if (own) {
+#pragma unroll
+        for (int v = 0; v < NEQ; ++v) {
+          L[v] = Fs[v];
+          R[v] = Fn[v];
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NEQ; ++v) {
+          L[v] = Fn[v];
+          R[v] = Fs[v];
+        }
+      }
+    } else {
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) L[v] = R[v] = Fs[v];
+      fk::ghost<NEQ, T>(mt, -nb, n, R);
+    }
+    fk::FaceRes<T, NEQ> fr;
+    fk::riemann<NEQ, T>(mt, L, R, n, nb == -5, fr);
+    if constexpr (NEQ == 5) {
+#pragma unroll
+      for (int v = 0; v < 5; ++v) rhs[v] += (double)(own ? -fr.f[v] * A : fr.f[v] * A);
+    } else {
+      const T sign = own ? (T)-1.0 : (T)1.0;
+      const T *u = fr.unc;
+      const T nc0 = avf * u[0] * n[0] + avf * u[1] * n[1] + avf * u[2] * n[2];
+      const T nc1 = avf * pLiq * u[0] * n[0] + avf * pLiq * u[1] * n[1] + avf * pLiq * u[2] * n[2];
+      const T nc2 = (1 - avf) * pGas * u[0] * n[0] + (1 - avf) * pGas * u[1] * n[1] + (1 - avf) * pGas * u[2] * n[2];
+      T Pc;
+      if constexpr (sizeof(T) == 8) Pc = fr.pF - (T)pCd;
+      else Pc = (T)((double)fr.pF - pCd);
+      rhs[0] += (double)(sign * (fr.f[0] - nc0) * A);
+      rhs[1] += (double)(sign * (fr.f[1]) * A);
+      rhs[2] += (double)(sign * (fr.f[2]) * A);
+      rhs[3] += (double)(sign * (fr.f[3] + Pc * n[0]) * A);
+      rhs[4] += (double)(sign * (fr.f[4] + Pc * n[1]) * A);
+      rhs[5] += (double)(sign * (fr.f[5] + Pc * n[2]) * A);
+      rhs[6] += (double)(sign * (fr.f[6] + nc1) * A);
+      rhs[7] += (double)(sign * (fr.f[7] + nc2) * A);
+      rhs[8] += (double)(sign * (fr.f[8]) * A);
+      rhs[9] += (double)(sign * (fr.f[9]) * A);
+    }
+    if constexpr (VISC) {
+      const T *Gc = gvel + c * NG;
+      const T *Gn = nb >= 0 ? gvel + (int64_t)nb * NG : Gc;
+      const T *G0 = own ? Gc : Gn;  // owner's gradient first, as (g_o + g_n)/2
+      const T *G1 = own ? Gn : Gc;
+      T gm[3][3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gm[i][d] = (G0[i * 3 + d] + G1[i * 3 + d]) / 2;
+      T eddy = 0;
+      if (P.turbulence == 1) eddy = fk::wale<T>(gm, wls[s]);
+      const T ux = gm[0][0], uy = gm[0][1], uz = gm[0][2], vx = gm[1][0], vy = gm[1][1], vz = gm[1][2];
+      const T wx = gm[2][0], wy = gm[2][1], wz = gm[2][2];
+      T st[3][3];
+      st[0][0] = (2 * ux - (T)(2.0 / 3.0) * (ux + vy + wz));
+      st[1][1] = (2 * vy - (T)(2.0 / 3.0) * (ux + vy + wz));
+      st[2][2] = (2 * wz - (T)(2.0 / 3.0) * (ux + vy + wz));
+      st[0][1] = st[1][0] = (uy + vx);
+      st[0][2] = st[2][0] = (uz + wx);
+      st[1][2] = st[2][1] = (vz + wy);
+      // side-1 state: reconstructed neighbour, or the first-order ghost on boundaries
+      T R1[NEQ];
+      if (nb >= 0) {
+#pragma unroll
+        for (int v = 0; v < NEQ; ++v) R1[v] = R[v];
+      } else {
+#pragma unroll
+        for (int v = 0; v < NEQ; ++v) R1[v] = Wc1[v];
+        fk::ghost<NEQ, T>(mt, -nb, n, R1);
+      }
+      T mu;
+      if constexpr (NEQ == 5) {
+        const T rhoF = (L[0] + R1[0]) / 2;
+        mu = mt.mu[0] + rhoF * eddy;
+      } else {
+        const T a1 = L[0], r1 = a1 * L[1] + (1 - a1) * L[2];
+        const T a2 = R1[0], r2 = a2 * R1[1] + (1 - a2) * R1[2];
+        const T rhoF = (r1 + r2) / 2, avfF = (a1 + a2) / 2;
+        mu = avfF * mt.mu[0] + (1 - avfF) * mt.mu[1] + rhoF * eddy;
+      }
+      T fm[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) fm[i] = (mu * st[i][0]) * n[0] + (mu * st[i][1]) * n[1] + (mu * st[i][2]) * n[2];
+      T vf[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) vf[i] = (L[iu + i] + R1[iu + i]) / 2;
+      T fe[3];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) {
+        T stv = 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) stv += st[i][d] * vf[d];
+        fe[i] = (mu * stv) * n[i];
+      }
+      const T en = fe[0] + fe[1] + fe[2];
+#pragma unroll
+      for (int i = 0; i < 3; ++i) rhs[iu + i] += (double)(own ? fm[i] * A : -(fm[i]) * A);
+      constexpr int ie = NEQ == 5 ? 4 : 8;
+      rhs[ie] += (double)(own ? en * A : -(en) * A);
+      if constexpr (HEAT) {
+        const T k = mt.cp[0] * mu / mt.Pr0;
+        T gf[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gf[d] = (G0[9 + d] + G1[9 + d]) / 2;
+        const T q = k * (gf[0] * n[0] + gf[1] * n[1] + gf[2] * n[2]);
+        rhs[4] += (double)(own ? q * A : -(q) * A);
+      }
+    }
+    if constexpr (MODE == 1) {
+      const T cl = fk::signal<NEQ, T>(mt, L, n);
+      const T cr = nb >= 0 ? fk::signal<NEQ, T>(mt, R, n) : (T)0.0;
+      const T cm = fmax(cl, cr);
+      if (!(sig >= cm)) sig = cm;
+    }
+  }
+  if constexpr (MODE == 1) {
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) rhs_out[c * NEQ + v] = rhs[v];
+    dtc[c] = volcfl[c] / (double)sig;
+  } else {
+    const double dt = dt_ptr ? *dt_ptr : dt_val;
+    const double Vc = vol[c];
+    constexpr int iz = NEQ == 5 ? 3 : 5;
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) {
+      const int neglect = (v == iz && P.neglectZ == 1);
+      double u;
+      if (P.rkSteps == 1) u = (1 - neglect) * (Uc[v] + dt * (1.0 / Vc) * rhs[v]);
+      else if (irk == 0) u = (1 - neglect) * (U0[c * NEQ + v] + dt * (1.0 / Vc) * rhs[v]);
+      else if (irk == 1)
+        u = (1 - neglect) * ((3.0 / 4.0) * U0[c * NEQ + v] + (1.0 / 4.0) * Uc[v] + (1.0 / 4.0) * dt * (1.0 / Vc) * rhs[v]);
+      else
+        u = (1 - neglect) * ((1.0 / 3.0) * U0[c * NEQ + v] + (2.0 / 3.0) * Uc[v] + (2.0 / 3.0) * dt * (1.0 / Vc) * rhs[v]);
+      Unew[c * NEQ + v] = u;
+    }
+  }
+}
+
+// RK stage from a stored residual (update.cu:19-103).
+template <int NEQ>
+__global__ void __launch_bounds__(kBlock) rk_update(fk_params P, int64_t nc, const double *__restrict__ U,
+                                                    const double *__restrict__ U0, double *__restrict__ Unew,
+                                                    const double *__restrict__ rhs, const double *__restrict__ vol,
+                                                    int irk, double dt_val, const double *__restrict__ dt_ptr) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nc * NEQ) return;
+  const int64_t c = i / NEQ;
+  const int v = (int)(i - c * NEQ);
+  const double dt = dt_ptr ? *dt_ptr : dt_val;
+  const double Vc = vol[c];
+  constexpr int iz = NEQ == 5 ? 3 : 5;
+  const int neglect = (v == iz && P.neglectZ == 1);
+  double u;
+  if (P.rkSteps == 1) u = (1 - neglect) * (U[i] + dt * (1.0 / Vc) * rhs[i]);
+  else if (irk == 0) u = (1 - neglect) * (U0[i] + dt * (1.0 / Vc) * rhs[i]);
+  else if (irk == 1) u = (1 - neglect) * ((3.0 / 4.0) * U0[i] + (1.0 / 4.0) * U[i] + (1.0 / 4.0) * dt * (1.0 / Vc) * rhs[i]);
+  else u = (1 - neglect) * ((1.0 / 3.0) * U0[i] + (2.0 / 3.0) * U[i] + (2.0 / 3.0) * dt * (1.0 / Vc) * rhs[i]);
+  Unew[i] = u;
+}
+
+// min over cells of Vcfl/signal: per-block minima, then one block folds them
+// and forms dt = min(cfl * min, maxDt) on the device.
+__global__ void __launch_bounds__(kBlock) min_blocks(int64_t n, const double *__restrict__ x, double *__restrict__ out) {
+  __shared__ double sh[kBlock];
+  double m = 1e15;
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) m = fmin(m, x[i]);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(kBlock) dt_final(int nb, const double *__restrict__ part, double cfl, double maxDt,
+                                                   double *__restrict__ scal) {
+  __shared__ double sh[kBlock];
+  double m = 1e15;
+  for (int i = threadIdx.x; i < nb; i += kBlock) m = fmin(m, part[i]);
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] = fmin(sh[threadIdx.x], sh[threadIdx.x + s]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) scal[0] = fmin(cfl * sh[0], maxDt);
+}
+
+// physical time += dt, once per step
+__global__ void advance_time(double *__restrict__ scal) {
+  if (threadIdx.x == 0) scal[1] += scal[0];
+}
+
+// ---------------- phase relaxation (relaxation.cu, thermRlx.cu, chemRlx.cu) ----------------
+
+__device__ __forceinline__ double clampd(const fk_params &P, double a) {
+  return fmin(fmax(P.avfMin / 100, a), 1.0 - P.avfMin / 100);
+}
+__device__ __forceinline__ double eos_p(const fk_params &P, int k, double rho, double e) {
+  return (P.gamma[k] - 1) * rho * e - P.gamma[k] * P.pInf[k] - (P.gamma[k] - 1) * P.eta[k] * rho;
+}
+__device__ __forceinline__ double eos_T(const fk_params &P, int k, double rho, double e) {
+  double p = (P.gamma[k] - 1) * rho * e - P.gamma[k] * P.pInf[k] - (P.gamma[k] - 1) * P.eta[k] * rho;
+  return (p + P.pInf[k]) / ((P.Cp[k] / P.gamma[k]) * (P.gamma[k] - 1) * rho);
+}
+
+__device__ double tsat_dev(const fk_params &P, double Pr, double temp, double urf, int *iters) {
+  const double Pinf = P.pInf[0], g1 = P.gamma[0], g2 = P.gamma[1], Cp1 = P.Cp[0], Cp2 = P.Cp[1];
+  const double Cv1 = Cp1 / g1, Cv2 = Cp2 / g2;
+  const double et2 = P.satConst[0] * log10(Pr) + P.satConst[1];
+  const double As = (Cp1 - Cp2 + et2 - 0.0) / (Cp2 - Cv2);
+  const double Bs = (P.eta[0] - P.eta[1]) / (Cp2 - Cv2);
+  const double Cs = (Cp2 - Cp1) / (Cp2 - Cv2);
+  const double Ds = (Cp1 - Cv1) / (Cp2 - Cv2);
+  const double dt = 1.e-3;
+  double fun = 1, T = temp;
+  int itr = 0;
+  while (fabs(fun) > 1.e-2 && itr < 50) {
+    fun = As + Bs / T + Cs * log10(T) + Ds * log10(Pr + Pinf) - log10(Pr);
+    double fd = As + Bs / (T + dt) + Cs * log10(T + dt) + Ds * log10(Pr + Pinf) - log10(Pr);
+    double gr = (fd - fun) / dt;
+    T = T - urf * fun / gr;
+    itr++;
+  }
+  *iters = itr;
+  return T;
+}
+
+struct ThermSys {
+  int pm;
+  double ar[2], enr[2];
+  __device__ void res(const fk_params &P, double x[4], double *r0, double *r1) const {
+    const int k = 1 - pm;
+    double s1 = ar[k] / x[k];
+    double s2 = ar[k] * (x[2 + k] - enr[k]);
+    x[pm] = (1 / (1 - s1)) * ar[pm];
+    x[2 + pm] = enr[pm] - (1 / ar[pm]) * s2;
+    double p0 = eos_p(P, 0, x[0], x[2]), p1 = eos_p(P, 1, x[1], x[3]);
+    double t0 = eos_T(P, 0, x[0], x[2]), t1 = eos_T(P, 1, x[1], x[3]);
+    *r0 = p0 - p1 - (double)0.0f;
+    *r1 = t0 - t1 - (double)0.0f;
+  }
+};
+
+__device__ int therm_dev(const fk_params &P, double *U) {
+  const int itrMax = P.thermItrMax;
+  const double urf = P.thermUrf;
+  double avf0 = clampd(P, U[0]);
+  double avf1 = 1 - avf0;
+  double rho0 = U[1] / avf0, rho1 = U[2] / avf1;
+  ThermSys t;
+  t.ar[0] = avf0 * rho0;
+  t.ar[1] = avf1 * rho1;
+  t.enr[0] = U[6] / t.ar[0];
+  t.enr[1] = U[7] / t.ar[1];
+  const double rhoMix = t.ar[0] + t.ar[1];
+  const double v0 = U[3] / rhoMix, v1 = U[4] / rhoMix, v2 = U[5] / rhoMix;
+  const double Y0 = t.ar[0] / rhoMix, Y1 = t.ar[1] / rhoMix;
+  double x[4] = {rho0, rho1, t.enr[0], t.enr[1]};
+  double ax[4] = {x[0], x[1], x[2], x[3]};
+  t.pm = (avf1 > avf0) ? 1 : 0;  // primary = larger volume fraction (thermRlx.cu:146-152)
+  const int pm = t.pm, k = 1 - pm;
+  const double de = U[8] / rhoMix - Y0 * t.enr[0] - Y1 * t.enr[1] - 0.5 * (v0 * v0 + v1 * v1 + v2 * v2);
+  t.enr[0] += de;
+  t.enr[1] += de;
+  int abortFlag = 0;
+  float minRes = 1e9f;
+  int itr = 0;
+  bool conv = false;
+  while (!conv) {
+    itr++;
+    double r[2];
+    t.res(P, x, &r[0], &r[1]);
+    double e[2], rd[2][2];
+    // perturb x[k] then x[k+2] (the non-primary unknowns in index order)
+    e[0] = 1e-4 * x[k];
+    x[k] += e[0];
+    t.res(P, x, &rd[0][0], &rd[1][0]);
+    x[k] -= e[0];
+    e[1] = 1e-4 * x[k + 2];
+    x[k + 2] += e[1];
+    t.res(P, x, &rd[0][1], &rd[1][1]);
+    x[k + 2] -= e[1];
+    double J[2][2];
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++) {
+        J[i][j] = (rd[i][j] - r[i]) / e[j];
+        if (isnan(J[i][j])) abortFlag = 1;
+      }
+    const double det = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+    double I00 = J[1][1] / det, I11 = J[0][0] / det, I10 = -J[1][0] / det, I01 = -J[0][1] / det;
+    if (det == 0) abortFlag = 1;
+    if (abortFlag == 0 && x[0] > 0.0 && x[1] > 0.0) {
+      float chk = (float)sqrt(r[0] * r[0] + r[1] * r[1]);
+      if (chk < minRes) {
+        minRes = chk;
+        for (int i = 0; i < 4; ++i) ax[i] = x[i];
+      }
+    }
+    double dx0 = 0;
+    dx0 += I00 * r[0];
+    dx0 += I01 * r[1];
+    x[k] = x[k] - urf * dx0;
+    double dx1 = 0;
+    dx1 += I10 * r[0];
+    dx1 += I11 * r[1];
+    x[k + 2] = x[k + 2] - urf * dx1;
+    if (!(fabs(r[0]) > 1.e-2) && !(fabs(r[1]) > 1.e-2)) conv = true;
+    if (abortFlag == 1 || itr > itrMax) {
+      for (int i = 0; i < 4; ++i) x[i] = ax[i];
+      conv = true;
+    }
+  }
+  const double s1 = t.ar[k] / x[k];
+  const double s2 = t.ar[k] * (x[2 + k] - t.enr[k]);
+  x[pm] = (1 / (1 - s1)) * t.ar[pm];
+  x[2 + pm] = t.enr[pm] - (1 / t.ar[pm]) * s2;
+  U[0] = clampd(P, t.ar[0] / x[0]);
+  U[6] = t.ar[0] * x[2];
+  U[7] = t.ar[1] * x[3];
+  return itr;
+}
+
+__device__ __forceinline__ double gibbs_dev(const fk_params &P, int k, double pr, double T) {
+  const double et = k == 0 ? 0.0 : P.satConst[0] * log10(pr) + P.satConst[1];
+  const double h = P.Cp[k] * T + P.eta[k];
+  const double s = (P.Cp[k] / P.gamma[k]) * log10(pow(T, P.gamma[k]) / pow(pr + P.pInf[k], P.gamma[k] - 1)) + et;
+  return h - T * s;
+}
+
+struct ChemSys {
+  int pm;
+  double rhoMix, rEint;
+  __device__ void res(const fk_params &P, double x[6], double *r0, double *r1, double *r2) const {
+    const int k = 1 - pm;
+    const double s1 = x[k], s2 = x[k] * x[k + 2], s3 = x[k] * x[k + 2] * x[k + 4];
+    x[pm] = 1 - s1;
+    x[pm + 2] = (rhoMix - s2) / x[pm];
+    x[pm + 4] = (rEint - s3) / (x[pm] * x[pm + 2]);
+    double pr[2], te[2], gi[2];
+    for (int q = 0; q < 2; q++) {
+      pr[q] = eos_p(P, q, x[q + 2], x[q + 4]);
+      te[q] = eos_T(P, q, x[q + 2], x[q + 4]);
+      gi[q] = gibbs_dev(P, q, pr[q], te[q]);
+    }
+    *r0 = pr[0] - pr[1];
+    *r1 = te[0] - te[1];
+    *r2 = gi[0] - gi[1];
+  }
+};
+
+__device__ void chk_update(const fk_params &P, double xOld, double *x, double dx, int i, float urf, int *abortFlag,
+                           double rho) {
+  if (i == 0 || i == 1) {
+    if (x[i] < 0)
+      while (x[i] < P.chemLimit && urf > 0.0005) {
+        urf = urf / 2;
+        x[i] = xOld - urf * dx;
+      }
+    if (x[i] > 1)
+      while (x[i] > 1.0 - P.chemLimit && urf > 0.0005) {
+        urf = urf / 2;
+        x[i] = xOld - urf * dx;
+      }
+    if (x[i] < 0) *abortFlag = 1;
+    if (x[i] > 1) *abortFlag = 1;
+  } else if (i == 2 || i == 3) {
+    double ar = x[i - 2] * x[i];
+    if (x[i] < 0)
+      while (x[i] < 0 && urf > 0.0005) {
+        urf = urf / 2;
+        x[i] = xOld - urf * dx;
+      }
+    if (ar > rho)
+      while (ar > rho && urf > 0.0005) {
+        urf = urf / 2;
+        x[i] = xOld - urf * dx;
+        ar = x[i - 2] * x[i];
+      }
+    if (x[i] < 0) *abortFlag = 1;
+    if (ar > rho) *abortFlag = 1;
+  }
+}
+
+__device__ int chem_dev(const fk_params &P, double *U) {
+  const int itrMax = P.chemItrMax;
+  const double urf = P.chemUrf;
+  double x[6], ax[6];
+  x[0] = clampd(P, U[0]);
+  x[1] = 1 - x[0];
+  x[2] = U[1] / x[0];
+  x[3] = U[2] / x[1];
+  x[4] = U[6] / (x[0] * x[2]);
+  x[5] = U[7] / (x[1] * x[3]);
+  for (int i = 0; i < 6; ++i) ax[i] = x[i];
+  ChemSys ch;
+  ch.rhoMix = x[0] * x[2] + x[1] * x[3];
+  const double u = U[3] / ch.rhoMix, v = U[4] / ch.rhoMix, w = U[5] / ch.rhoMix;
+  ch.rEint = U[8] - 0.5 * ch.rhoMix * (u * u + v * v + w * w);
+  ch.pm = (x[1] > x[0]) ? 1 : 0;
+  const int pm = ch.pm;
+  int idx[3];  // non-primary unknowns in index order
+  {
+    int q = 0;
+    for (int j = 0; j < 6; ++j)
+      if (j != pm && j != pm + 2 && j != pm + 4) idx[q++] = j;
+  }
+  int abortFlag = 0;
+  float minRes = 1e9f;
+  int itr = 0;
+  bool conv = false;
+  while (!conv) {
+    itr++;
+    double r[3];
+    ch.res(P, x, &r[0], &r[1], &r[2]);
+    double e[3], rd[3][3];
+    for (int q = 0; q < 3; ++q) {
+      const int j = idx[q];
+      e[q] = fabs(x[j]) < 1e-9 ? 1e-10 : 1.e-5 * x[j];
+      x[j] += e[q];
+      ch.res(P, x, &rd[0][q], &rd[1][q], &rd[2][q]);
+      x[j] -= e[q];
+    }
+    double J[3][3], I[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        J[i][j] = (rd[i][j] - r[i]) / e[j];
+        if (isnan(J[i][j])) abortFlag = 1;
+      }
+    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    if (det == 0) abortFlag = 1;
+    const double id = 1.0 / det;
+    I[0][0] = id * (J[1][1] * J[2][2] - J[1][2] * J[2][1]);
+    I[0][1] = -id * (J[0][1] * J[2][2] - J[0][2] * J[2][1]);
+    I[0][2] = id * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
+    I[1][0] = -id * (J[1][0] * J[2][2] - J[1][2] * J[2][0]);
+    I[1][1] = id * (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
+    I[1][2] = -id * (J[0][0] * J[1][2] - J[0][2] * J[1][0]);
+    I[2][0] = id * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    I[2][1] = -id * (J[0][0] * J[2][1] - J[0][1] * J[2][0]);
+    I[2][2] = id * (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+    if (abortFlag == 0 && x[0] > 0.0 && x[0] < 1.0 && x[1] > 0.0 && x[1] < 1.0 && x[2] > 0.0 && x[3] > 0.0) {
+      float chk = (float)sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+      if (chk < minRes) {
+        minRes = chk;
+        for (int i = 0; i < 6; ++i) ax[i] = x[i];
+      }
+    }
+    int abortLocal = abortFlag;
+    for (int q = 0; q < 3; ++q) {
+      const int i = idx[q];
+      double dx = 0;
+      for (int j = 0; j < 3; j++) dx += I[q][j] * r[j];
+      x[i] = x[i] - urf * dx;
+      chk_update(P, x[i] + urf * dx, x, dx, i, (float)urf, &abortLocal, ch.rhoMix);
+    }
+    if (!(fabs(r[0]) > 1.e-1) && !(fabs(r[1]) > 1.e-1) && !(fabs(r[2]) > 1.e-1)) conv = true;
+    if (abortLocal == 1 || itr > itrMax) {
+      for (int i = 0; i < 6; ++i) x[i] = ax[i];
+      conv = true;
+    }
+  }
+  const int k = 1 - pm;
+  const double s1 = x[k], s2 = x[k] * x[k + 2], s3 = x[k] * x[k + 2] * x[k + 4];
+  x[pm] = 1 - s1;
+  x[pm + 2] = (ch.rhoMix - s2) / x[pm];
+  x[pm + 4] = (ch.rEint - s3) / (x[pm] * x[pm + 2]);
+  U[0] = clampd(P, x[0]);
+  U[1] = x[0] * x[2];
+  U[2] = x[1] * x[3];
+  U[6] = x[0] * x[2] * x[4];
+  U[7] = x[1] * x[3] * x[5];
+  U[9] = x[1] * x[3];
+  return itr;
+}
+
+// partitionRelaxation + thermalRlx/chemRlx in one pass: each thread
+// classifies its cell and runs the matching Newton solve in place.
+// counts: [n_chem, newton iterations, tsat iterations] (64-bit).
+__global__ void __launch_bounds__(kBlock) relax_plain(fk_params P, int64_t nc, double *__restrict__ U,
+                                                      unsigned long long *__restrict__ counts) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  int chem = 0, it = 0, ts = 0;
+  if (c < nc) {
+    double u[10];
+    for (int v = 0; v < 10; ++v) u[v] = U[c * 10 + v];
+    if (P.chemFlag == 1) {
+      const double avf = clampd(P, u[0]);
+      const double rho = u[1] / avf;
+      const double enr = u[6] / (avf * rho);
+      const double tL = eos_T(P, 0, rho, enr);
+      const double pL = eos_p(P, 0, rho, enr);
+      const double Ts = tsat_dev(P, pL, tL, P.satTempUrf, &ts);
+      chem = (tL > Ts && avf > P.chemLimit) ? 1 : 0;
+    }
+    it = chem ? chem_dev(P, u) : therm_dev(P, u);
+    for (int v = 0; v < 10; ++v) U[c * 10 + v] = u[v];
+  }
+  // wave-level sums, one atomic per wave
+  unsigned long long a = chem, b = it, d = ts;
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+    d += __shfl_down(d, off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&counts[0], a);
+    atomicAdd(&counts[1], b);
+    atomicAdd(&counts[2], d);
+  }
+}
+
+// sum_c U_i V_c per block (consCalc.cu:3-34); i = 0 integrates 1 - alpha.
+template <int NEQ>
+__global__ void __launch_bounds__(kBlock) integrate(int64_t nc, const double *__restrict__ U,
+                                                    const double *__restrict__ vol, double *__restrict__ part) {
+  __shared__ double sh[kBlock];
+  for (int v = 0; v < NEQ; ++v) {
+    double s = 0;
+    for (int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x; c < nc; c += (int64_t)gridDim.x * kBlock) {
+      double u = U[c * NEQ + v];
+      if (NEQ == 10 && v == 0) u = 1 - u;
+      s += u * vol[c];
+    }
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = kBlock / 2; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x * NEQ + v] = sh[0];
+    __syncthreads();
+  }
+}
+
+__global__ void zero_ull(unsigned long long *p, int n) {
+  int i = threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// ------------------------------------------------------------------------
+// host helpers
+// ------------------------------------------------------------------------
+
+inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+inline size_t tsz(const fk_ctx *c) { return c->p.precision == FK_MIXED ? 4 : 8; }
+inline int ng_of(const fk_ctx *c) { return 9 + ((c->p.model == FK_MODEL_SINGLE && c->p.heatFlux == 1) ? 3 : 0); }
+
+int timer_id(fk_ctx *c, const char *name) {
+  for (size_t i = 0; i < c->tm.names.size(); ++i)
+    if (c->tm.names[i] == name) return (int)i;
+  c->tm.names.emplace_back(name);
+  c->tm.counts.push_back(0);
+  c->tm.ms.push_back(0);
+  return (int)c->tm.names.size() - 1;
+}
+
+int timer_begin(fk_ctx *c, const char *name, int *slot) {
+  *slot = -1;
+  if (!c->timing) return FK_OK;
+  Timer::Pending pd;
+  pd.id = timer_id(c, name);
+  if (c->tm.pool.size() >= 2) {
+    pd.b = c->tm.pool.back();
+    c->tm.pool.pop_back();
+    pd.a = c->tm.pool.back();
+    c->tm.pool.pop_back();
+  } else {
+    HIPCHK(hipEventCreate(&pd.a));
+    HIPCHK(hipEventCreate(&pd.b));
+  }
+  HIPCHK(hipEventRecord(pd.a, c->stream));
+  c->tm.pending.push_back(pd);
+  *slot = (int)c->tm.pending.size() - 1;
+  return FK_OK;
+}
+int timer_end(fk_ctx *c, int slot) {
+  if (slot < 0) return FK_OK;
+  HIPCHK(hipEventRecord(c->tm.pending[(size_t)slot].b, c->stream));
+  return FK_OK;
+}
+int timer_flush(fk_ctx *c) {
+  for (auto &pd : c->tm.pending) {
+    HIPCHK(hipEventSynchronize(pd.b));
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, pd.a, pd.b));
+    c->tm.counts[(size_t)pd.id]++;
+    c->tm.ms[(size_t)pd.id] += ms;
+    c->tm.pool.push_back(pd.a);
+    c->tm.pool.push_back(pd.b);
+  }
+  c->tm.pending.clear();
+  return FK_OK;
+}
+
+#define LAUNCH_TIMED(ctx, name, ...)                  \
+  do {                                                \
+    int slot_;                                        \
+    int rc_ = timer_begin(ctx, name, &slot_);         \
+    if (rc_) return rc_;                              \
+    __VA_ARGS__;                                      \
+    HIPCHK(hipGetLastError());                        \
+    rc_ = timer_end(ctx, slot_);                      \
+    if (rc_) return rc_;                              \
+  } while (0)
+
+int next_buffer(const fk_ctx *c) {
+  for (int i = 0; i < 3; ++i)
+    if (i != c->ia && i != c->iu0) return i;
+  return (c->ia + 1) % 3;
+}
+
+int require(const fk_ctx *c) {
+  if (!c) return fail(FK_EINVAL, "null context");
+  if (!c->have_mesh) return fail(FK_ESTATE, "no mesh uploaded");
+  if (!c->have_state) return fail(FK_ESTATE, "no state uploaded");
+  return FK_OK;
+}
+
+// upload the precision-dependent slot geometry
+int upload_geometry(fk_ctx *c) {
+  const int64_t ns = c->nc * 4;
+  if (c->p.precision == FK_MIXED) {
+    std::vector<float> g((size_t)ns * 4), r((size_t)ns * 4), l((size_t)ns);
+    for (size_t i = 0; i < g.size(); ++i) g[i] = (float)c->h_geo[i];
+    for (size_t i = 0; i < r.size(); ++i) r[i] = (float)c->h_r[i];
+    for (size_t i = 0; i < l.size(); ++i) l[i] = (float)c->h_ls[i];
+    HIPCHK(hipMemcpy(c->geo, g.data(), g.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->rvec, r.data(), r.size() * 4, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->wls, l.data(), l.size() * 4, hipMemcpyHostToDevice));
+  } else {
+    HIPCHK(hipMemcpy(c->geo, c->h_geo.data(), c->h_geo.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->rvec, c->h_r.data(), c->h_r.size() * 8, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(c->wls, c->h_ls.data(), c->h_ls.size() * 8, hipMemcpyHostToDevice));
+  }
+  return FK_OK;
+}
+
+void free_all(fk_ctx *c) {
+  void *ptrs[] = {c->nbr, c->mir, c->geo, c->rvec, c->wls, c->vol, c->volcfl, c->U[0], c->U[1], c->U[2], c->face,
+                  c->gvel, c->rhs, c->dtc, c->red, c->dscal, c->dgrad, c->dtheta, c->lcount};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  c->nbr = c->mir = nullptr;
+  c->geo = c->rvec = c->wls = nullptr;
+  c->vol = c->volcfl = nullptr;
+  c->U[0] = c->U[1] = c->U[2] = nullptr;
+  c->face = c->gvel = nullptr;
+  c->rhs = c->dtc = c->red = c->dscal = c->dgrad = c->dtheta = nullptr;
+  c->lcount = nullptr;
+}
+
+template <typename T>
+int launch_pass_g_t(fk_ctx *c) {
+  const unsigned gr = grid_for(c->nc);
+  const bool visc = c->p.viscFlag == 1;
+  const bool heat = visc && c->p.model == FK_MODEL_SINGLE && c->p.heatFlux == 1;
+  double *dg = c->debug ? c->dgrad : nullptr;
+  double *dth = c->debug ? c->dtheta : nullptr;
+  const double *U = c->U[c->ia];
+  T *face = (T *)c->face;
+  T *gv = (T *)c->gvel;
+#define PG(NEQ, VISC, HEAT)                                                                                            \
+  LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g<NEQ, T, VISC, HEAT>), dim3(gr), dim3(kBlock), 0, c->stream, c->p, \
+                                               c->nc, U, c->nbr, c->mir, c->geo, c->rvec, c->vol, face, gv, dg, dth))
+  if (c->neq == 5) {
+    if (heat) PG(5, true, true);
+    else if (visc) PG(5, true, false);
+    else PG(5, false, false);
+  } else {
+    if (visc) PG(10, true, false);
+    else PG(10, false, false);
+  }
+#undef PG
+  return FK_OK;
+}
+
+template <typename T, int MODE>
+int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Unew) {
+  const unsigned gr = grid_for(c->nc);
+  const bool visc = c->p.viscFlag == 1;
+  const bool heat = visc && c->p.model == FK_MODEL_SINGLE && c->p.heatFlux == 1;
+  const double *U = c->U[c->ia], *U0 = c->U[c->iu0];
+  const T *face = (const T *)c->face;
+  const T *gv = (const T *)c->gvel;
+  const T *wl = (const T *)c->wls;
+  const char *nm = MODE == 0 ? "pass_f" : "pass_f_rhs";
+#define PF(NEQ, VISC, HEAT)                                                                                       \
+  LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE>), dim3(gr), dim3(kBlock), 0, c->stream, \
+                                         c->p, c->nc, U, U0, Unew, c->nbr, c->mir, c->geo, wl, c->vol, c->volcfl,  \
+                                         face, gv, c->rhs, c->dtc, irk, dt, dtp))
+  if (c->neq == 5) {
+    if (heat) PF(5, true, true);
+    else if (visc) PF(5, true, false);
+    else PF(5, false, false);
+  } else {
+    if (visc) PF(10, true, false);
+    else PF(10, false, false);
+  }
+#undef PF
+  return FK_OK;
+}
+
+int do_pass_g(fk_ctx *c) {
+  return c->p.precision == FK_MIXED ? launch_pass_g_t<float>(c) : launch_pass_g_t<double>(c);
+}
+
+int do_pass_f_rhs(fk_ctx *c) {
+  int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 1>(c, 0, 0.0, nullptr, nullptr)
+                                      : launch_pass_f_t<double, 1>(c, 0, 0.0, nullptr, nullptr);
+  if (rc) return rc;
+  c->rhs_valid = true;
+  return FK_OK;
+}
+
+int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp) {
+  const int nb = next_buffer(c);
+  int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 0>(c, irk, dt, dtp, c->U[nb])
+                                      : launch_pass_f_t<double, 0>(c, irk, dt, dtp, c->U[nb]);
+  if (rc) return rc;
+  c->ia = nb;
+  c->rhs_valid = false;
+  return FK_OK;
+}
+
+int do_update(fk_ctx *c, int irk, double dt, const double *dtp) {
+  const int nb = next_buffer(c);
+  const unsigned gr = grid_for(c->nc * c->neq);
+  if (c->neq == 5)
+    LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update<5>), dim3(gr), dim3(kBlock), 0, c->stream, c->p, c->nc,
+                                                    c->U[c->ia], c->U[c->iu0], c->U[nb], c->rhs, c->vol, irk, dt, dtp));
+  else
+    LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update<10>), dim3(gr), dim3(kBlock), 0, c->stream, c->p, c->nc,
+                                                    c->U[c->ia], c->U[c->iu0], c->U[nb], c->rhs, c->vol, irk, dt, dtp));
+  c->ia = nb;
+  return FK_OK;
+}
+
+int do_dt_device(fk_ctx *c) {
+  const int nb = 1024;
+  LAUNCH_TIMED(c, "dt_reduce",
+               hipLaunchKernelGGL(min_blocks, dim3(nb), dim3(kBlock), 0, c->stream, c->nc, c->dtc, c->red);
+               hipLaunchKernelGGL(dt_final, dim3(1), dim3(kBlock), 0, c->stream, nb, c->red, c->p.cfl, c->p.maxDt,
+                                  c->dscal));
+  return FK_OK;
+}
+
+int do_relax(fk_ctx *c, fk_relax_stats *st) {
+  if (c->p.model != FK_MODEL_TWO_PHASE) return fail(FK_EINVAL, "relaxation needs the two-phase model");
+  LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_ull, dim3(1), dim3(64), 0, c->stream, (unsigned long long *)c->lcount, 4));
+  LAUNCH_TIMED(c, "relax", hipLaunchKernelGGL(relax_plain, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, c->p,
+                                              c->nc, c->U[c->ia], (unsigned long long *)c->lcount));
+  if (st) {
+    unsigned long long h[4];
+    HIPCHK(hipMemcpyAsync(h, c->lcount, sizeof h, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(hipStreamSynchronize(c->stream));
+    st->n_chem = (int64_t)h[0];
+    st->n_therm = c->nc - (int64_t)h[0];
+    st->newton_iters = (int64_t)h[1];
+    st->tsat_iters = (int64_t)h[2];
+    st->n_fallback = 0;
+  }
+  return FK_OK;
+}
+
+int check_params(const fk_params *p) {
+  if (!p) return fail(FK_EINVAL, "null params");
+  if (p->model != FK_MODEL_SINGLE && p->model != FK_MODEL_TWO_PHASE) return fail(FK_EINVAL, "bad model");
+  if (p->precision != FK_FP64 && p->precision != FK_MIXED) return fail(FK_EINVAL, "bad precision");
+  if (p->rkSteps != 1 && p->rkSteps != 3) return fail(FK_EINVAL, "rkSteps must be 1 or 3");
+  if (p->adaptDt == 1 && p->adaptDtItr < 1) return fail(FK_EINVAL, "adaptDtItr must be >= 1");
+  return FK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *fk_last_error(void) { return g_err.c_str(); }
+
+int fk_create(int device, const fk_params *p, fk_ctx **out) {
+  if (!out) return fail(FK_EINVAL, "null out");
+  *out = nullptr;
+  int rc = check_params(p);
+  if (rc) return rc;
+  int ndev = 0;
+  HIPCHK(hipGetDeviceCount(&ndev));
+  if (device < 0 || device >= ndev) return fail(FK_EDEVICE, "no such HIP device");
+  HIPCHK(hipSetDevice(device));
+  fk_ctx *c = new (std::nothrow) fk_ctx;
+  if (!c) return fail(FK_ENOMEM, "out of host memory");
+  c->device = device;
+  c->p = *p;
+  c->neq = p->model == FK_MODEL_SINGLE ? 5 : 10;
+  hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete c;
+    return fail(FK_EDEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  *out = c;
+  return FK_OK;
+}
+
+int fk_destroy(fk_ctx *c) {
+  if (!c) return FK_OK;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->stream);
+  for (auto &pd : c->tm.pending) {
+    (void)hipEventDestroy(pd.a);
+    (void)hipEventDestroy(pd.b);
+  }
+  for (auto e : c->tm.pool) (void)hipEventDestroy(e);
+  free_all(c);
+  (void)hipStreamDestroy(c->stream);
+  delete c;
+  return FK_OK;
+}
+
+int fk_set_params(fk_ctx *c, const fk_params *p) {
+  if (!c) return fail(FK_EINVAL, "null context");
+  int rc = check_params(p);
+  if (rc) return rc;
+  if (p->model != c->p.model) return fail(FK_EINVAL, "the model cannot change on a context");
+  HIPCHK(hipSetDevice(c->device));
+  const bool prec_changed = p->precision != c->p.precision;
+  const bool heat_changed = (p->heatFlux != c->p.heatFlux) || (p->viscFlag != c->p.viscFlag);
+  c->p = *p;
+  if (c->have_mesh && (prec_changed || heat_changed)) {
+    // the T-typed arrays were sized for double and for the largest NG; only refill geometry
+    rc = upload_geometry(c);
+    if (rc) return rc;
+  }
+  c->rhs_valid = false;
+  return FK_OK;
+}
+
+int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el, const int32_t *boundCond,
+                   const double *n, const double *area, const double *fcCenter, const double *cellCenter,
+                   const double *volume, const double *shift) {
+  if (!c || !fc2el || !boundCond || !n || !area || !fcCenter || !cellCenter || !volume)
+    return fail(FK_EINVAL, "null argument");
+  if (nElem <= 0 || nFcs <= 0) return fail(FK_EINVAL, "empty mesh");
+  if (nElem * 4 >= (int64_t)1 << 30) return fail(FK_ERANGE, "mesh too large for 30-bit slot ids");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  // cell -> faces in ascending face id
+  std::vector<int32_t> sf((size_t)(nElem * 4), -1);
+  std::vector<uint8_t> cnt((size_t)nElem, 0);
+  for (int64_t f = 0; f < nFcs; ++f) {
+    const int32_t o = fc2el[2 * f], nb = fc2el[2 * f + 1];
+    if (o < 0 || o >= nElem || nb >= nElem || nb == o) return fail(FK_EINVAL, "bad fc2el entry");
+    const int bc = boundCond[f];
+    if ((nb < 0) != (bc != 0)) return fail(FK_EINVAL, "boundCond must be nonzero exactly on boundary faces");
+    if (bc < 0 || bc > 5) return fail(FK_EINVAL, "boundCond must be 0..5");
+    for (int s = 0; s < 2; ++s) {
+      const int32_t cc = fc2el[2 * f + s];
+      if (cc < 0) continue;
+      if (cnt[(size_t)cc] >= 4) return fail(FK_EINVAL, "cell with more than four faces (only tetrahedra)");
+      sf[(size_t)(cc * 4 + cnt[(size_t)cc]++)] = (int32_t)f;
+    }
+  }
+  for (int64_t cc = 0; cc < nElem; ++cc)
+    if (cnt[(size_t)cc] != 4) return fail(FK_EINVAL, "every cell must have exactly four faces (tetrahedra)");
+  const int64_t ns = nElem * 4;
+  std::vector<int32_t> hnbr((size_t)ns), hmir((size_t)ns);
+  c->h_geo.assign((size_t)ns * 4, 0.0);
+  c->h_r.assign((size_t)ns * 4, 0.0);
+  c->h_ls.assign((size_t)ns, 0.0);
+  // face -> slot of the owner side / neighbour side
+  std::vector<int32_t> fslot((size_t)(nFcs * 2), -1);
+  for (int64_t s = 0; s < ns; ++s) {
+    const int64_t cc = s / 4;
+    const int32_t f = sf[(size_t)s];
+    const int side = fc2el[2 * f] == cc ? 0 : 1;
+    fslot[(size_t)(2 * f + side)] = (int32_t)s;
+  }
+  for (int64_t s = 0; s < ns; ++s) {
+    const int64_t cc = s / 4;
+    const int32_t f = sf[(size_t)s];
+    const int side = fc2el[2 * f] == cc ? 0 : 1;
+    const int32_t other = fc2el[2 * f + (1 - side)];
+    if (other >= 0) {
+      hnbr[(size_t)s] = other;
+      hmir[(size_t)s] = (fslot[(size_t)(2 * f + 1 - side)] << 1) | (side == 0 ? 1 : 0);
+    } else {
+      hnbr[(size_t)s] = -boundCond[f];
+      hmir[(size_t)s] = 1;
+    }
+    for (int d = 0; d < 3; ++d) c->h_geo[(size_t)(s * 4 + d)] = n[3 * f + d];
+    c->h_geo[(size_t)(s * 4 + 3)] = area[f];
+    for (int d = 0; d < 3; ++d) {
+      double xf = fcCenter[3 * f + d];
+      if (side == 1 && shift) xf = xf - shift[3 * f + d];
+      c->h_r[(size_t)(s * 4 + d)] = xf - cellCenter[3 * cc + d];
+    }
+    const int32_t a = fc2el[2 * f];
+    const int32_t bb = fc2el[2 * f + 1] >= 0 ? fc2el[2 * f + 1] : a;
+    c->h_ls[(size_t)s] = 0.325 * std::pow((volume[a] + volume[bb]) / 2, 0.3333333333333);
+  }
+  std::vector<double> vcfl((size_t)nElem);
+  for (int64_t cc = 0; cc < nElem; ++cc) vcfl[(size_t)cc] = std::pow(volume[cc], 0.3333);
+  free_all(c);
+  c->have_mesh = c->have_state = false;
+  c->nc = nElem;
+  c->nf = nFcs;
+  const int neq = c->neq;
+  const size_t cs = (size_t)nElem;
+  HIPCHK(hipMalloc(&c->nbr, (size_t)ns * 4));
+  HIPCHK(hipMalloc(&c->mir, (size_t)ns * 4));
+  HIPCHK(hipMalloc(&c->geo, (size_t)ns * 4 * 8));
+  HIPCHK(hipMalloc(&c->rvec, (size_t)ns * 4 * 8));
+  HIPCHK(hipMalloc(&c->wls, (size_t)ns * 8));
+  HIPCHK(hipMalloc(&c->vol, cs * 8));
+  HIPCHK(hipMalloc(&c->volcfl, cs * 8));
+  for (int i = 0; i < 3; ++i) HIPCHK(hipMalloc(&c->U[i], cs * neq * 8));
+  HIPCHK(hipMalloc(&c->face, (size_t)ns * neq * 8));
+  HIPCHK(hipMalloc(&c->gvel, cs * 12 * 8));
+  HIPCHK(hipMalloc(&c->rhs, cs * neq * 8));
+  HIPCHK(hipMalloc(&c->dtc, cs * 8));
+  HIPCHK(hipMalloc(&c->red, 4096 * 8));
+  HIPCHK(hipMalloc(&c->dscal, 16 * 8));
+  HIPCHK(hipMalloc(&c->dgrad, cs * neq * 3 * 8));
+  HIPCHK(hipMalloc(&c->dtheta, cs * neq * 8));
+  HIPCHK(hipMalloc(&c->lcount, 8 * 8));
+  HIPCHK(hipMemcpy(c->nbr, hnbr.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->mir, hmir.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->vol, volume, cs * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->volcfl, vcfl.data(), cs * 8, hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(c->dscal, 0, 16 * 8));
+  c->slot_face.swap(sf);
+  int rc = upload_geometry(c);
+  if (rc) return rc;
+  c->have_mesh = true;
+  c->rhs_valid = false;
+  return FK_OK;
+}
+
+int fk_upload_state(fk_ctx *c, const double *U) {
+  if (!c || !U) return fail(FK_EINVAL, "null argument");
+  if (!c->have_mesh) return fail(FK_ESTATE, "upload the mesh first");
+  HIPCHK(hipSetDevice(c->device));
+  c->ia = c->iu0 = 0;
+  HIPCHK(hipMemcpy(c->U[0], U, (size_t)c->nc * c->neq * 8, hipMemcpyHostToDevice));
+  c->have_state = true;
+  c->rhs_valid = false;
+  return FK_OK;
+}
+
+int fk_download_state(fk_ctx *c, double *U) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!U) return fail(FK_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  HIPCHK(hipMemcpy(U, c->U[c->ia], (size_t)c->nc * c->neq * 8, hipMemcpyDeviceToHost));
+  return FK_OK;
+}
+
+int fk_init_rk(fk_ctx *c) {
+  int rc = require(c);
+  if (rc) return rc;
+  c->iu0 = c->ia;
+  return FK_OK;
+}
+
+int fk_compute_face_values(fk_ctx *c) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  rc = do_pass_g(c);
+  c->rhs_valid = false;
+  return rc;
+}
+
+int fk_compute_rhs(fk_ctx *c) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  return do_pass_f_rhs(c);
+}
+
+int fk_compute_time_step(fk_ctx *c, double *dt) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!c->rhs_valid) return fail(FK_ESTATE, "fk_compute_rhs must run before fk_compute_time_step");
+  HIPCHK(hipSetDevice(c->device));
+  rc = do_dt_device(c);
+  if (rc) return rc;
+  double h[2];
+  HIPCHK(hipMemcpyAsync(h, c->dscal, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  c->dt = h[0];
+  if (dt) *dt = h[0];
+  return FK_OK;
+}
+
+int fk_update(fk_ctx *c, int irk, double dt) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!c->rhs_valid) return fail(FK_ESTATE, "fk_compute_rhs must run before fk_update");
+  if (irk < 0 || irk >= c->p.rkSteps) return fail(FK_EINVAL, "bad RK stage");
+  HIPCHK(hipSetDevice(c->device));
+  rc = do_update(c, irk, dt, nullptr);
+  c->rhs_valid = false;
+  return rc;
+}
+
+int fk_compute_rhs_update(fk_ctx *c, int irk, double dt) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (irk < 0 || irk >= c->p.rkSteps) return fail(FK_EINVAL, "bad RK stage");
+  HIPCHK(hipSetDevice(c->device));
+  return do_pass_f_update(c, irk, dt, nullptr);
+}
+
+int fk_relaxation(fk_ctx *c, int mode, fk_relax_stats *st) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (mode != FK_RELAX_PLAIN) return fail(FK_EINVAL, "relaxation mode not available");
+  HIPCHK(hipSetDevice(c->device));
+  return do_relax(c, st);
+}
+
+int fk_cons_calc(fk_ctx *c, double *out) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!out) return fail(FK_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  const int nb = 256;
+  if (c->neq == 5)
+    hipLaunchKernelGGL((integrate<5>), dim3(nb), dim3(kBlock), 0, c->stream, c->nc, c->U[c->ia], c->vol, c->red);
+  else
+    hipLaunchKernelGGL((integrate<10>), dim3(nb), dim3(kBlock), 0, c->stream, c->nc, c->U[c->ia], c->vol, c->red);
+  HIPCHK(hipGetLastError());
+  std::vector<double> h((size_t)nb * c->neq);
+  HIPCHK(hipMemcpyAsync(h.data(), c->red, h.size() * 8, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  for (int v = 0; v < c->neq; ++v) {
+    double s = 0;
+    for (int b = 0; b < nb; ++b) s += h[(size_t)(b * c->neq + v)];
+    out[v] = s;
+  }
+  return FK_OK;
+}
+
+int fk_run_steps(fk_ctx *c, int nsteps, double *dt_last, double *phys_time) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (nsteps < 0) return fail(FK_EINVAL, "negative step count");
+  HIPCHK(hipSetDevice(c->device));
+  const fk_params &p = c->p;
+  // the device scalars hold dt and the physical time
+  double h[2] = {c->dt, c->time};
+  HIPCHK(hipMemcpy(c->dscal, h, sizeof h, hipMemcpyHostToDevice));
+  for (int s = 0; s < nsteps; ++s) {
+    if (p.rkSteps > 1) c->iu0 = c->ia;
+    const bool newdt = p.adaptDt == 1 && c->step % p.adaptDtItr == 0;
+    for (int irk = 0; irk < p.rkSteps; ++irk) {
+      if ((rc = do_pass_g(c))) return rc;
+      if (irk == 0 && newdt) {
+        if ((rc = do_pass_f_rhs(c))) return rc;
+        if ((rc = do_dt_device(c))) return rc;
+        if ((rc = do_update(c, irk, 0.0, c->dscal))) return rc;
+      } else if (p.adaptDt == 1) {
+        if ((rc = do_pass_f_update(c, irk, 0.0, c->dscal))) return rc;
+      } else {
+        if ((rc = do_pass_f_update(c, irk, p.dt, nullptr))) return rc;
+      }
+    }
+    if (p.adaptDt == 1)
+      LAUNCH_TIMED(c, "advance_time", hipLaunchKernelGGL(advance_time, dim3(1), dim3(64), 0, c->stream, c->dscal));
+    else
+      c->dt = p.dt;
+    if (p.model == FK_MODEL_TWO_PHASE && p.relaxFlag == 1)
+      if ((rc = do_relax(c, nullptr))) return rc;
+    c->step++;
+  }
+  HIPCHK(hipMemcpyAsync(h, c->dscal, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (p.adaptDt == 1) {
+    c->dt = h[0];
+    c->time = h[1];
+  } else {
+    c->time += p.dt * nsteps;
+  }
+  if (dt_last) *dt_last = c->dt;
+  if (phys_time) *phys_time = c->time;
+  return FK_OK;
+}
+
+int fk_sync(fk_ctx *c) {
+  if (!c) return fail(FK_EINVAL, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  return FK_OK;
+}
+
+int fk_set_debug(fk_ctx *c, int on) {
+  if (!c) return fail(FK_EINVAL, "null context");
+  c->debug = on != 0;
+  return FK_OK;
+}
+
+int fk_download(fk_ctx *c, const char *name, void *host, int64_t bytes) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!name || !host) return fail(FK_EINVAL, "null argument");
+  HIPCHK(hipSetDevice(c->device));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  std::string s(name);
+  const int64_t nc = c->nc, neq = c->neq;
+  const size_t ts = tsz(c);
+  auto get = [&](const void *src, int64_t nb) -> int {
+    if (nb != bytes) return fail(FK_EINVAL, "size mismatch for " + s + ": expected " + std::to_string(nb));
+    HIPCHK(hipMemcpy(host, src, (size_t)nb, hipMemcpyDeviceToHost));
+    return FK_OK;
+  };
+  if (s == "grad") return get(c->dgrad, nc * neq * 3 * 8);
+  if (s == "theta") return get(c->dtheta, nc * neq * 8);
+  if (s == "face") return get(c->face, nc * 4 * neq * (int64_t)ts);
+  if (s == "rhs") return get(c->rhs, nc * neq * 8);
+  if (s == "signal") {
+    // dtc holds Vcfl / signal; return the signal itself
+    std::vector<double> d((size_t)nc), v((size_t)nc);
+    if (bytes != nc * 8) return fail(FK_EINVAL, "size mismatch for signal");
+    HIPCHK(hipMemcpy(d.data(), c->dtc, (size_t)nc * 8, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(v.data(), c->volcfl, (size_t)nc * 8, hipMemcpyDeviceToHost));
+    double *o = (double *)host;
+    for (int64_t i = 0; i < nc; ++i) o[i] = v[(size_t)i] / d[(size_t)i];
+    return FK_OK;
+  }
+  if (s == "slots") {
+    if (bytes != nc * 4 * 4) return fail(FK_EINVAL, "size mismatch for slots");
+    std::memcpy(host, c->slot_face.data(), (size_t)bytes);
+    return FK_OK;
+  }
+  return fail(FK_EINVAL, "unknown array " + s);
+}
+
+int fk_set_timing(fk_ctx *c, int on) {
+  if (!c) return fail(FK_EINVAL, "null context");
+  c->timing = on != 0;
+  return FK_OK;
+}
+
+int fk_kernel_times(fk_ctx *c, int max, char (*names)[32], int64_t *counts, double *ms) {
+  if (!c) return fail(FK_EINVAL, "null context");
+  HIPCHK(hipSetDevice(c->device));
+  int rc = timer_flush(c);
+  if (rc) return rc;
+  const int n = (int)c->tm.names.size();
+  for (int i = 0; i < n && i < max; ++i) {
+    if (names) {
+      std::strncpy(names[i], c->tm.names[(size_t)i].c_str(), 31);
+      names[i][31] = 0;
+    }
+    if (counts) counts[i] = c->tm.counts[(size_t)i];
+    if (ms) ms[i] = c->tm.ms[(size_t)i];
+  }
+  return n;
+}
+
+}  // extern "C"

@@ -1,0 +1,149 @@
+/*
+ * fooska.h -- C ABI of the MI355X finite-volume step (libfooska_hip.so).
+ *
+ * One context = one device = one HIP stream; calls on a context must be
+ * externally synchronised.  The caller owns host buffers, the library owns
+ * device buffers.  Every function returns FK_OK (0) or a negative FK_E* code
+ * (fooska_mesh.h); fk_last_error() returns the message of the last error of
+ * the calling thread.
+ *
+ * The entry points replace the reference's host "phase" functions, which are
+ * declared in mulitphase Solver/doublePrecision/strdata.h:206-247 and called
+ * in this order by explicitLoop (explicitLoop.cu:28-64):
+ *   per RK stage: computePrimVec -> computeGrad -> computeFaceValues ->
+ *                 computeRHS -> [computeTimeStep at stage 0] -> update
+ *   per step:     relaxation, consCalc
+ * The first three are fused into fk_compute_face_values (pass G) and the
+ * last two into fk_compute_rhs_update (pass F).  Face-to-cell accumulation
+ * is a deterministic per-cell gather over the cell's faces in ascending face
+ * order (the reference uses floating-point atomics), so repeated runs are
+ * bitwise identical.
+ */
+#ifndef FOOSKA_H
+#define FOOSKA_H
+#include <stdint.h>
+
+#include "fooska_mesh.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FK_MODEL_SINGLE 0    /* 5 equations: rho, rho u, rho v, rho w, rho E (SP/) */
+#define FK_MODEL_TWO_PHASE 1 /* 10 equations of mulitphase Solver (DP/, MP/) */
+
+#define FK_FP64 0  /* every operation in double precision */
+#define FK_MIXED 1 /* FP32 reconstruction, HLLC, WALE and geometry;
+                      FP64 state, RK accumulation, EOS inversion, Newton */
+
+#define FK_RELAX_PLAIN 0 /* reference Newton solves (relaxation.cu, thermRlx.cu, chemRlx.cu) */
+#define FK_RELAX_MLP 1   /* MLP warm start + FP64 Newton polish, plain Newton as fallback */
+
+/* Solver and material constants; field names follow the reference INPUT
+ * struct (strdata.h:49-98).  Two-phase: index 0 = liquid, 1 = gas.
+ * Single-phase uses index 0 only. */
+typedef struct fk_params {
+  int32_t model;      /* FK_MODEL_* */
+  int32_t precision;  /* FK_FP64 / FK_MIXED */
+  int32_t rkSteps;    /* 1 (forward Euler) or 3 (Shu-Osher SSP-RK3) */
+  int32_t adaptDt;    /* 1: dt from the CFL condition every adaptDtItr steps */
+  int32_t adaptDtItr;
+  int32_t viscFlag;   /* Newtonian viscous stresses */
+  int32_t turbulence; /* WALE eddy viscosity (Cw = 0.325) */
+  int32_t heatFlux;   /* Fourier heat flux k = Cp mu_eff / Pr (single-phase; no reference counterpart) */
+  int32_t woodsFlag;  /* Wood mixture sound speed where rho Y_v > chemLimit */
+  int32_t neglectX, neglectY, neglectZ;
+  int32_t chemFlag;   /* classify cells and run the chemical (p-T-g) relaxation */
+  int32_t thermItrMax, chemItrMax;
+  int32_t hllcEnergy; /* 0: energy star state as in starRegion.h:32, 1: Toro's form */
+  int32_t relaxFlag;  /* two-phase: run the per-step phase relaxation */
+  int32_t pad_;
+  double dt;          /* fixed time step (adaptDt == 0) */
+  double cfl, maxDt, avfMin;
+  double thermUrf, chemUrf, chemLimit, satTempUrf;
+  double polishTol;   /* relative tolerance of the FP64 Newton polish */
+  double gamma[2], pInf[2], Cp[2], eta[2], satConst[2], dynVisc[2], Pr[2];
+} fk_params;
+
+typedef struct fk_ctx fk_ctx;
+
+typedef struct fk_relax_stats {
+  int64_t n_therm, n_chem;      /* cells classified thermal / chemical */
+  int64_t newton_iters;         /* Newton iterations summed over cells (therm + chem) */
+  int64_t tsat_iters;           /* saturation-temperature iterations summed */
+  int64_t n_fallback;           /* warm-started cells that fell back to the plain solve */
+} fk_relax_stats;
+
+const char *fk_last_error(void);
+
+/* Context on HIP device `device`. */
+int fk_create(int device, const fk_params *p, fk_ctx **out);
+int fk_destroy(fk_ctx *ctx);
+/* Replace the constants (model and NEQ must stay the same once a mesh is up). */
+int fk_set_params(fk_ctx *ctx, const fk_params *p);
+
+/* Mesh in the reference MESH semantics (strdata.h:100-115, one rank):
+ * fc2el[nFcs][2] (owner < neighbour, -1 on boundary faces), boundCond[nFcs],
+ * n[nFcs][3] unit normal out of the owner, area[nFcs], fcCenter[nFcs][3],
+ * cellCenter[nElem][3], volume[nElem]; shift[nFcs][3] (periodic translation
+ * of the face as seen from the neighbour) may be NULL.  Only tetrahedra:
+ * every cell must have exactly four faces. */
+int fk_upload_mesh(fk_ctx *ctx, int64_t nElem, int64_t nFcs, const int32_t *fc2el, const int32_t *boundCond,
+                   const double *n, const double *area, const double *fcCenter, const double *cellCenter,
+                   const double *volume, const double *shift);
+
+/* Conserved state, reference AoS layout consVec[nElem][NEQ]. */
+int fk_upload_state(fk_ctx *ctx, const double *consVec);
+int fk_download_state(fk_ctx *ctx, double *consVec);
+
+/* ---- phase functions (reference order) ---- */
+/* initRk (update.cu:141): keep the stage-0 state for the SSP-RK3 stages. */
+int fk_init_rk(fk_ctx *ctx);
+/* computePrimVec + computeGrad + computeFaceValues (computePrimVec.cu:82,
+ * computeGrad.cu:156, computeFaceValues.cu:138): first-order face states,
+ * Green-Gauss gradients, neighbour min/max, limiter and MUSCL face states. */
+int fk_compute_face_values(fk_ctx *ctx);
+/* computeRHS (computeRHS.cu:214): HLLC + non-conservative terms + pressure
+ * correction (+ viscous/WALE) gathered per cell; kept on the device for
+ * fk_update.  Also records the per-cell signal speed for fk_compute_time_step. */
+int fk_compute_rhs(fk_ctx *ctx);
+/* computeTimeStep (computeTimeStep.cu:172): dt = min(cfl * min_c V^0.3333 /
+ * max signal, maxDt) from the face states of the last fk_compute_rhs. */
+int fk_compute_time_step(fk_ctx *ctx, double *dt);
+/* update (update.cu:105): RK stage irk with the residual of fk_compute_rhs. */
+int fk_update(fk_ctx *ctx, int irk, double dt);
+/* computeRHS + update fused: same result as fk_compute_rhs + fk_update. */
+int fk_compute_rhs_update(fk_ctx *ctx, int irk, double dt);
+/* relaxation (relaxation.cu:77): per-cell phase relaxation, mode FK_RELAX_*. */
+int fk_relaxation(fk_ctx *ctx, int mode, fk_relax_stats *stats);
+/* consCalc (consCalc.cu:36): out[i] = sum_c U_i V_c (i = 0 integrates 1 - alpha). */
+int fk_cons_calc(fk_ctx *ctx, double *out);
+
+/* Whole time steps in explicitLoop order (dt is taken before the stage-0
+ * update of every adaptDtItr-th step).  Returns the last dt and the time. */
+int fk_run_steps(fk_ctx *ctx, int nsteps, double *dt_last, double *phys_time);
+
+/* Block until the context's stream is idle. */
+int fk_sync(fk_ctx *ctx);
+
+/* Download an intermediate array for phase-level parity checks.  Names and
+ * layouts (FP64 path; the mixed path stores FP32 where marked f):
+ *  "grad"   [nElem][NEQ][3]   Green-Gauss gradients (needs fk_set_debug(1))
+ *  "theta"  [nElem][NEQ]      limiter (needs fk_set_debug(1))
+ *  "face"   [nElem][4][NEQ]   reconstructed face states per cell face slot (f)
+ *  "rhs"    [nElem][NEQ]      residual of the last fk_compute_rhs (f)
+ *  "slots"  [nElem][4] int32  face id of each cell face slot
+ *  "signal" [nElem]           max signal speed per cell                      */
+int fk_download(fk_ctx *ctx, const char *name, void *host, int64_t bytes);
+int fk_set_debug(fk_ctx *ctx, int on);
+
+/* Per-kernel timing with HIP events on the context's stream.  When on, every
+ * launch is bracketed by events; fk_kernel_times reports, for each named
+ * kernel, the launch count and the summed milliseconds. */
+int fk_set_timing(fk_ctx *ctx, int on);
+int fk_kernel_times(fk_ctx *ctx, int max, char (*names)[32], int64_t *counts, double *ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,164 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU FP64 oracle
+on the same seeded inputs.
+
+Bars (north star): FP64 path <= 1e-12 relative L1 per conserved variable
+after N steps; mixed path <= 1e-5 relative L1 in density.  Phase-level
+intermediates of the FP64 path are expected to agree to rounding (the
+arithmetic order is the same; only libm transcendentals in the relaxation
+differ between glibc and the device library)."""
+import numpy as np
+import pytest
+
+from fooskagpu_amd import GpuSolver, _lib, cases
+from oracle.pyoracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+FP64_TOL = 1e-12
+MIXED_RHO_TOL = 1e-5
+
+
+def rel_l1(a, b):
+    """Relative L1 error per column: sum|a-b| / sum|b|."""
+    a = np.asarray(a, dtype=np.float64).reshape(len(a), -1)
+    b = np.asarray(b, dtype=np.float64).reshape(len(b), -1)
+    den = np.abs(b).sum(0)
+    return np.abs(a - b).sum(0) / np.where(den > 0, den, 1.0)
+
+
+def cavitation(n=10):
+    """Mixture cells with alpha_L = 0.5 in liquid superheated with respect to
+    Tsat(1500 Pa) ~ 286 K: exercises the chemical (p-T-g) relaxation."""
+    c = cases.bubble(n=n)
+    p = c.params
+    regions = [dict(kind=0, alphaL=1 - p["avfMin"], p=1e5, T=300.0),
+               dict(kind=1, c=(0.5, 0.5, 0.5), r=0.3, alphaL=0.5, p=1500.0, T=300.0)]
+    c.U = c.mesh.init_regions(p, regions)
+    c.name = "cavitation"
+    return c
+
+
+SMALL = {
+    "sod": lambda: cases.sod(nx=16, ny=4, nz=4),
+    "tgv": lambda: cases.tgv(n=8),
+    "bubble": lambda: cases.bubble(n=10),
+    "shock_bubble": lambda: cases.shock_bubble(n=10, nbub=3),
+    "cavitation": lambda: cavitation(10),
+}
+
+
+def neq_density(case):
+    """Columns holding density: rho (single-phase) or alpha_k rho_k (two-phase)."""
+    return [0] if case.neq == 5 else [1, 2]
+
+
+@pytest.fixture(params=sorted(SMALL))
+def case(request):
+    c = SMALL[request.param]()
+    print(f"[case] {c.name} cells={c.mesh.n_cells}", flush=True)
+    return c
+
+
+def test_face_values_and_rhs_match_oracle(gpu, case):
+    """Pass G (grad, theta, MUSCL face states) and pass F (residual, signal)
+    on the initial state, FP64."""
+    o = Oracle(case.params, case.mesh, case.U)
+    with GpuSolver(case.params, case.mesh, case.U) as g:
+        g.set_debug(True)
+        g.compute_face_values()
+        o.compute_face_values()
+        slots_g, slots_o = g.download("slots"), o.get("slots")
+        assert np.array_equal(slots_g, slots_o)
+        for name in ("grad", "theta", "face"):
+            a, b = g.download(name), o.get(name)
+            err = np.max(np.abs(a - b) / np.maximum(np.abs(b), 1e-300))
+            print(f"[{case.name}] {name}: max rel diff {err:.3e} bitwise={np.array_equal(a, b)}", flush=True)
+            assert err <= 1e-13, name
+        g.compute_rhs()
+        o.compute_rhs()
+        for name in ("rhs", "signal"):
+            a, b = g.download(name), o.get(name)
+            e = rel_l1(a.reshape(len(a), -1), b.reshape(len(b), -1)).max()
+            print(f"[{case.name}] {name}: rel L1 {e:.3e} bitwise={np.array_equal(a, b)}", flush=True)
+            assert e <= 1e-13, name
+        dt_g, dt_o = g.compute_time_step(), o.compute_time_step()
+        assert abs(dt_g - dt_o) <= 1e-14 * dt_o
+
+
+def test_fp64_steps_match_oracle(gpu, case):
+    steps = 10
+    o = Oracle(case.params, case.mesh, case.U)
+    o.run_steps(steps)
+    with GpuSolver(case.params, case.mesh, case.U) as g:
+        g.run_steps(steps)
+        Ug = g.download_state()
+    Uo = o.state()
+    assert np.isfinite(Ug).all()
+    err = rel_l1(Ug, Uo)
+    print(f"[{case.name}] FP64 {steps} steps rel L1 per variable: {np.array2string(err, precision=2)}", flush=True)
+    assert err.max() <= FP64_TOL
+
+
+def test_phase_api_equals_fused_loop(gpu, case):
+    """explicitLoop phase by phase (fk_compute_rhs + fk_update) and the fused
+    device loop (fk_run_steps) give bitwise identical states."""
+    with GpuSolver(case.params, case.mesh, case.U) as a, GpuSolver(case.params, case.mesh, case.U) as b:
+        a.explicit_loop_phases(4)
+        b.run_steps(4)
+        assert np.array_equal(a.download_state(), b.download_state())
+
+
+def test_repeat_runs_bitwise_identical(gpu, case):
+    out = []
+    for _ in range(2):
+        with GpuSolver(case.params, case.mesh, case.U) as g:
+            g.run_steps(5)
+            out.append(g.download_state())
+    assert np.array_equal(out[0], out[1])
+
+
+def test_mixed_precision_density_error(gpu, case):
+    steps = 10
+    o = Oracle(case.params, case.mesh, case.U)
+    o.run_steps(steps)
+    Uo = o.state()
+    p = dict(case.params, precision=_lib.MIXED)
+    with GpuSolver(p, case.mesh, case.U) as g:
+        g.run_steps(steps)
+        Ug = g.download_state()
+    assert np.isfinite(Ug).all()
+    err = rel_l1(Ug, Uo)
+    cols = neq_density(case)
+    rho_err = rel_l1(Ug[:, cols].sum(1, keepdims=True), Uo[:, cols].sum(1, keepdims=True))[0]
+    print(f"[{case.name}] mixed {steps} steps: density rel L1 {rho_err:.2e}; per variable "
+          f"{np.array2string(err, precision=2)}", flush=True)
+    assert rho_err <= MIXED_RHO_TOL
+
+
+def test_relaxation_matches_oracle(gpu):
+    """Plain Newton relaxation (partition + thermal / chemical) on a hydro-
+    evolved cavitation state."""
+    c = cavitation(10)
+    p = dict(c.params, relaxFlag=0)
+    o = Oracle(p, c.mesh, c.U)
+    o.run_steps(3)
+    U = o.state()
+    st_o = o.relaxation()
+    with GpuSolver(c.params, c.mesh, U) as g:
+        st_g = g.relaxation()
+        Ug = g.download_state()
+    Uo = o.state()
+    print(f"oracle {st_o} gpu {st_g}", flush=True)
+    assert st_o["n_chem"] > 0
+    assert st_g["n_chem"] == st_o["n_chem"]
+    err = rel_l1(Ug, Uo)
+    assert err.max() <= FP64_TOL, err
+
+
+def test_cons_calc_matches_oracle(gpu):
+    c = SMALL["bubble"]()
+    o = Oracle(c.params, c.mesh, c.U)
+    with GpuSolver(c.params, c.mesh, c.U) as g:
+        a = g.cons_calc()
+    b = o.cons_calc()
+    assert np.allclose(a, b, rtol=1e-12, atol=0)
