@@ -48,10 +48,19 @@ def relax_bytes(neq):
     return 8 * neq + 48
 
 
-def run_gpu(case, precision, steps, warmup, device=0):
-    from fooskagpu_amd import GpuSolver
-    p = dict(case.params, precision=precision)
+def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None):
+    """Time `steps` explicitLoop steps on the device (state resident in HBM).
+    train: dict of surrogate-training options (done before the timed region)."""
+    from fooskagpu_amd import GpuSolver, surrogate
+    p = dict(case.params, precision=precision, relaxMode=0)
     g = GpuSolver(p, case.mesh, case.U, device=device)
+    info = {}
+    if train is not None:
+        t0 = time.perf_counter()
+        _, (l0, l1, n) = surrogate.train_on_solver(g, case.U, case.params, log=log, **train)
+        info = {"train_s": time.perf_counter() - t0, "loss_first": l0, "loss_last": l1, "train_samples": n}
+        log(f"surrogate trained on {n} states in {info['train_s']:.1f} s, loss {l0:.4f} -> {l1:.6f}")
+    g.set_params(dict(p, relaxMode=relax_mode))
     g.sync()
     if warmup:
         g.run_steps(warmup)
@@ -64,7 +73,7 @@ def run_gpu(case, precision, steps, warmup, device=0):
     kt = g.kernel_times()
     U = g.download_state()
     g.close()
-    return el, kt, U, dt
+    return el, kt, U, dt, info
 
 
 def cpu_baseline(target_s=15.0):
@@ -95,6 +104,11 @@ def main():
     ap.add_argument("--n", type=int, default=200, help="hexes per axis (200 -> 48M tets)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-mlp", action="store_true", help="skip the surrogate run")
+    ap.add_argument("--order", default="morton", choices=["rcm", "morton", "none"], help="cell order of the mesh")
+    ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--copy-bytes", type=int, default=1 << 30,
+                    help="size of the plain device copy measured as the HBM reference (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (halo exchange is out of scope); "
@@ -103,24 +117,37 @@ def main():
 
     from fooskagpu_amd import FP64, MIXED, cases
     t0 = time.perf_counter()
-    case = cases.shock_bubble(n=args.n, nbub=64)
+    case = cases.shock_bubble(n=args.n, nbub=64, order=args.order)
     C = case.mesh.n_cells
     fc = case.mesh.n_faces / C
     log(f"mesh {C} cells, {case.mesh.n_faces} faces (F/C {fc:.4f}) built in {time.perf_counter() - t0:.1f} s")
     s = 3
     res = {}
-    for name, prec in (("mixed", MIXED), ("fp64", FP64)):
-        if name == "fp64" and args.no_fp64:
+    copy = None
+    if args.copy_bytes:
+        from fooskagpu_amd import GpuSolver
+        with GpuSolver(dict(case.params)) as g0:
+            ms = min(g0.stream_copy(args.copy_bytes) for _ in range(3))
+        copy = {"bytes": args.copy_bytes, "ms": ms, "GBps": 2 * args.copy_bytes / ms / 1e6}
+        log(f"stream_copy {args.copy_bytes} B: {ms:.3f} ms, {copy['GBps']:.0f} GB/s (read + write)")
+    from fooskagpu_amd import RELAX_MLP, RELAX_PLAIN
+    ntrain = min(4_000_000, C)
+    train = dict(n_solver=ntrain, n_uniform=ntrain // 4, warm_steps=3, train_steps=args.train_steps)
+    plan = [("mixed_mlp", MIXED, RELAX_MLP, train), ("mixed", MIXED, RELAX_PLAIN, None),
+            ("fp64", FP64, RELAX_PLAIN, None)]
+    for name, prec, mode, tr in plan:
+        if (name == "fp64" and args.no_fp64) or (name == "mixed_mlp" and args.no_mlp):
             continue
         log(f"{name}: warmup {args.warmup}, timing {args.steps} steps")
-        el, kt, U, dt = run_gpu(case, prec, args.steps, args.warmup)
+        el, kt, U, dt, info = run_gpu(case, prec, args.steps, args.warmup, relax_mode=mode, train=tr)
         rate = C * s * args.steps / el
         log(f"{name}: {el:.3f} s, {rate:.4g} cell-stage/s, last dt {dt:.3e}, finite={bool(np.isfinite(U).all())}")
         for k, (n, ms) in sorted(kt.items(), key=lambda kv: -kv[1][1]):
             log(f"   {k:14s} {n:5d} launches {ms:9.2f} ms  avg {ms / max(n, 1):8.3f} ms")
-        res[name] = dict(el=el, rate=rate, kt=kt, U=U)
+        res[name] = dict(el=el, rate=rate, kt=kt, U=U, info=info)
 
-    mixed = res["mixed"]
+    head = "mixed_mlp" if "mixed_mlp" in res else "mixed"
+    mixed = res[head]
     sb = 4
     B = stage_bytes(10, s, fc, sb, sb)
     kt = mixed["kt"]
@@ -142,13 +169,21 @@ def main():
         "dtype": "fp32+fp64",
         "data": "synthetic",
         "config": {"workload": f"shock-bubble {args.n}^3 jittered hexes x 6 tets ({C} cells), 64 vapour bubbles, "
-                               f"WALE, SSP-RK3, CFL 0.1, plain Newton relaxation, mixed precision",
-                   "cells": C, "faces": case.mesh.n_faces},
+                               f"WALE, SSP-RK3, CFL 0.1, mixed precision, "
+                               + ("MLP warm start + FP64 Newton polish" if head == "mixed_mlp" else
+                                  "plain Newton relaxation"),
+                   "cells": C, "faces": case.mesh.n_faces, "cell_order": args.order},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
                      "kernel": "pass_f", "bytes_per_cell": B["f_u0"]},
         "whole_step_hbm_frac": whole / HBM_PEAK,
+        "stream_copy": copy,
     }
+    if head == "mixed_mlp":
+        out["surrogate_training"] = mixed["info"]
+    for other in ("mixed",):
+        if other in res and other != head:
+            out[other] = {"value": res[other]["rate"], "ms_per_step": res[other]["el"] / args.steps * 1e3}
     if "fp64" in res:
         f64 = res["fp64"]
         Bd = stage_bytes(10, s, fc, 8, 8)

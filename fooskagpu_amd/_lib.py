@@ -25,7 +25,7 @@ class FkParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int32) for n in (
         "model", "precision", "rkSteps", "adaptDt", "adaptDtItr", "viscFlag", "turbulence",
         "heatFlux", "woodsFlag", "neglectX", "neglectY", "neglectZ", "chemFlag", "thermItrMax",
-        "chemItrMax", "hllcEnergy", "relaxFlag", "pad_")] + [(n, ctypes.c_double) for n in (
+        "chemItrMax", "hllcEnergy", "relaxFlag", "relaxMode")] + [(n, ctypes.c_double) for n in (
         "dt", "cfl", "maxDt", "avfMin", "thermUrf", "chemUrf", "chemLimit", "satTempUrf",
         "polishTol")] + [(n, ctypes.c_double * 2) for n in (
         "gamma", "pInf", "Cp", "eta", "satConst", "dynVisc", "Pr")]
@@ -115,6 +115,19 @@ HIP_API = [
     ("fk_sync", ctypes.c_int, [_ctx]),
     ("fk_download", ctypes.c_int, [_ctx, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64]),
     ("fk_set_debug", ctypes.c_int, [_ctx, ctypes.c_int]),
+    ("fk_stream_copy", ctypes.c_int, [_ctx, ctypes.c_int64, c_double_p]),
+    ("fk_mlp_set_weights", ctypes.c_int, [_ctx, ctypes.c_void_p, ctypes.c_void_p]),
+    ("fk_mlp_get_weights", ctypes.c_int, [_ctx, ctypes.c_void_p, ctypes.c_void_p]),
+    ("fk_mlp_set_norm", ctypes.c_int, [_ctx, ctypes.c_void_p]),
+    ("fk_mlp_label", ctypes.c_int, [_ctx, c_double_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_void_p]),
+    ("fk_mlp_data", ctypes.c_int, [_ctx, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, ctypes.c_int]),
+    ("fk_mlp_emit", ctypes.c_int, [_ctx, ctypes.c_int64, ctypes.c_int64]),
+    ("fk_mlp_download_data", ctypes.c_int, [_ctx, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int64, c_int64_p]),
+    ("fk_mlp_grad", ctypes.c_int, [_ctx, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p,
+                                   ctypes.POINTER(ctypes.c_float)]),
+    ("fk_mlp_train", ctypes.c_int, [_ctx, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_uint64,
+                                    ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)]),
+    ("fk_mlp_infer", ctypes.c_int, [_ctx, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]),
     ("fk_set_timing", ctypes.c_int, [_ctx, ctypes.c_int]),
     ("fk_kernel_times", ctypes.c_int, [_ctx, ctypes.c_int, ctypes.c_void_p, c_int64_p, c_double_p]),
 ]

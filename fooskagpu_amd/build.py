@@ -45,7 +45,7 @@ def build_mesh():
 def build_hip():
     os.makedirs(LIB, exist_ok=True)
     out = os.path.join(LIB, "libfooska_hip.so")
-    srcs = [os.path.join(CSRC, "fk_kernels.hip")]
+    srcs = [os.path.join(CSRC, "fk_kernels.hip"), os.path.join(CSRC, "fk_mlp.hip")]
     _run([HIPCC] + HIP_FLAGS + srcs + ["-o", out + ".tmp"])
     os.replace(out + ".tmp", out)
     return out

@@ -34,7 +34,7 @@ WATER_VAPOUR = dict(
 BASE = dict(
     model=_lib.MODEL_SINGLE, precision=_lib.FP64, rkSteps=3, adaptDt=1, adaptDtItr=1, viscFlag=0,
     turbulence=0, heatFlux=0, woodsFlag=1, neglectX=0, neglectY=0, neglectZ=0, chemFlag=1,
-    thermItrMax=25, chemItrMax=40, hllcEnergy=0, relaxFlag=1, pad_=0,
+    thermItrMax=25, chemItrMax=40, hllcEnergy=0, relaxFlag=1, relaxMode=0,
     dt=0.0, cfl=0.5, maxDt=1e30, avfMin=1e-4, thermUrf=0.8, chemUrf=0.5, chemLimit=20e-4,
     satTempUrf=0.8, polishTol=1e-10,
     gamma=[1.4, 1.4], pInf=[0.0, 0.0], Cp=[1006.0, 1006.0], eta=[0.0, 0.0], satConst=[0.0, 0.0],
@@ -106,12 +106,12 @@ def bubble(n=128, steps=100, seed=2, rcm=True):
     return Case("bubble", p, m, U, steps)
 
 
-def shock_bubble(n=200, steps=50, seed=4, nbub=64, rcm=True):
+def shock_bubble(n=200, steps=50, seed=4, nbub=64, rcm=True, order=None):
     """Planar 5e7 Pa liquid shock (rho 1005) entering from x = 0 into liquid
     at 1e5 Pa holding nbub vapour bubbles (radii U[0.02, 0.05], seeded,
     non-overlapping, x >= 0.2); WALE on."""
     p = _two_phase_params(viscFlag=1, turbulence=1)
-    m = Mesh.box(n, n, n, (0, 0, 0), (1, 1, 1), 0.10, seed, bc=(BC_OUTLET,) * 6, rcm=rcm)
+    m = Mesh.box(n, n, n, (0, 0, 0), (1, 1, 1), 0.10, seed, bc=(BC_OUTLET,) * 6, rcm=rcm, order=order)
     am = p["avfMin"]
     gL, piL, cpL = p["gamma"][0], p["pInf"][0], p["Cp"][0]
     cvL = cpL / gL

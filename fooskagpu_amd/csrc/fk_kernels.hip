@@ -24,6 +24,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -34,68 +35,10 @@
 
 #include "../../include/fooska.h"
 #include "fk_device.h"
+#include "fk_relax.h"
+#include "fk_internal.h"
 
-namespace {
-
-thread_local std::string g_err;
-int fail(int code, const std::string &m) {
-  g_err = m;
-  return code;
-}
-#define HIPCHK(x)                                                                              \
-  do {                                                                                         \
-    hipError_t e_ = (x);                                                                       \
-    if (e_ != hipSuccess) return fail(FK_EDEVICE, std::string(#x ": ") + hipGetErrorString(e_)); \
-  } while (0)
-
-constexpr int kBlock = 256;
-
-template <typename T>
-struct V4 {
-  T x, y, z, w;
-};
-
-struct Timer {
-  std::vector<std::string> names;
-  std::vector<int64_t> counts;
-  std::vector<double> ms;
-  struct Pending {
-    int id;
-    hipEvent_t a, b;
-  };
-  std::vector<Pending> pending;
-  std::vector<hipEvent_t> pool;
-};
-
-}  // namespace
-
-struct fk_ctx {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  fk_params p{};
-  int neq = 5;
-  int64_t nc = 0, nf = 0;
-  bool have_mesh = false, have_state = false, debug = false, timing = false;
-  // mesh
-  int32_t *nbr = nullptr, *mir = nullptr;
-  void *geo = nullptr, *rvec = nullptr, *wls = nullptr;  // T-typed, sized for double
-  double *vol = nullptr, *volcfl = nullptr;
-  std::vector<int32_t> slot_face;  // host copy of slot -> face id
-  // fields
-  double *U[3] = {nullptr, nullptr, nullptr};
-  int ia = 0, iu0 = 0;
-  void *face = nullptr, *gvel = nullptr;
-  double *rhs = nullptr, *dtc = nullptr, *red = nullptr, *dscal = nullptr;
-  double *dgrad = nullptr, *dtheta = nullptr;
-  int *icount = nullptr;
-  long long *lcount = nullptr;
-  double dt = 0, time = 0;
-  int64_t step = 0;
-  bool rhs_valid = false;
-  Timer tm;
-  // host geometry kept to rebuild the precision-dependent arrays
-  std::vector<double> h_geo, h_r, h_ls;
-};
+using namespace fkx;
 
 namespace {
 
@@ -114,6 +57,16 @@ constexpr int ngrad() {
   return 9 + (HEAT ? 3 : 0);
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (each with its own
+// L2).  Remap so that XCD x processes the contiguous block range
+// [x G/8, (x+1) G/8): neighbouring cells (close in the mesh order) then share
+// an L2.  G is a multiple of 8 (padded by grid_xcd).
+constexpr int kXcd = 8;
+__device__ __forceinline__ int64_t xcd_block() {
+  const int64_t b = blockIdx.x, per = gridDim.x / kXcd;
+  return (b % kXcd) * per + b / kXcd;
+}
+
 template <typename T>
 __device__ __forceinline__ V4<T> ld4(const void *p, int64_t i) {
   return reinterpret_cast<const V4<T> *>(p)[i];
@@ -129,7 +82,7 @@ __global__ void __launch_bounds__(kBlock) pass_g(fk_params P, int64_t nc, const 
                                                  const double *__restrict__ vol, T *__restrict__ face,
                                                  T *__restrict__ gvel, double *__restrict__ dgrad,
                                                  double *__restrict__ dtheta) {
-  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t c = xcd_block() * kBlock + threadIdx.x;
   if (c >= nc) return;
   const fk::Mat<double> md = fk::make_mat<double>(P);
   const fk::Mat<T> mt = fk::make_mat<T>(P);
@@ -248,7 +201,7 @@ __global__ void __launch_bounds__(kBlock) pass_f(fk_params P, int64_t nc, const 
                                                  const T *__restrict__ face, const T *__restrict__ gvel,
                                                  double *__restrict__ rhs_out, double *__restrict__ dtc, int irk,
                                                  double dt_val, const double *__restrict__ dt_ptr) {
-  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int64_t c = xcd_block() * kBlock + threadIdx.x;
   if (c >= nc) return;
   const fk::Mat<double> md = fk::make_mat<double>(P);
   const fk::Mat<T> mt = fk::make_mat<T>(P);
@@ -489,293 +442,16 @@ __global__ void advance_time(double *__restrict__ scal) {
   if (threadIdx.x == 0) scal[1] += scal[0];
 }
 
-// ---------------- phase relaxation (relaxation.cu, thermRlx.cu, chemRlx.cu) ----------------
-
-__device__ __forceinline__ double clampd(const fk_params &P, double a) {
-  return fmin(fmax(P.avfMin / 100, a), 1.0 - P.avfMin / 100);
-}
-__device__ __forceinline__ double eos_p(const fk_params &P, int k, double rho, double e) {
-  return (P.gamma[k] - 1) * rho * e - P.gamma[k] * P.pInf[k] - (P.gamma[k] - 1) * P.eta[k] * rho;
-}
-__device__ __forceinline__ double eos_T(const fk_params &P, int k, double rho, double e) {
-  double p = (P.gamma[k] - 1) * rho * e - P.gamma[k] * P.pInf[k] - (P.gamma[k] - 1) * P.eta[k] * rho;
-  return (p + P.pInf[k]) / ((P.Cp[k] / P.gamma[k]) * (P.gamma[k] - 1) * rho);
-}
-
-__device__ double tsat_dev(const fk_params &P, double Pr, double temp, double urf, int *iters) {
-  const double Pinf = P.pInf[0], g1 = P.gamma[0], g2 = P.gamma[1], Cp1 = P.Cp[0], Cp2 = P.Cp[1];
-  const double Cv1 = Cp1 / g1, Cv2 = Cp2 / g2;
-  const double et2 = P.satConst[0] * log10(Pr) + P.satConst[1];
-  const double As = (Cp1 - Cp2 + et2 - 0.0) / (Cp2 - Cv2);
-  const double Bs = (P.eta[0] - P.eta[1]) / (Cp2 - Cv2);
-  const double Cs = (Cp2 - Cp1) / (Cp2 - Cv2);
-  const double Ds = (Cp1 - Cv1) / (Cp2 - Cv2);
-  const double dt = 1.e-3;
-  double fun = 1, T = temp;
-  int itr = 0;
-  while (fabs(fun) > 1.e-2 && itr < 50) {
-    fun = As + Bs / T + Cs * log10(T) + Ds * log10(Pr + Pinf) - log10(Pr);
-    double fd = As + Bs / (T + dt) + Cs * log10(T + dt) + Ds * log10(Pr + Pinf) - log10(Pr);
-    double gr = (fd - fun) / dt;
-    T = T - urf * fun / gr;
-    itr++;
-  }
-  *iters = itr;
-  return T;
-}
-
-struct ThermSys {
-  int pm;
-  double ar[2], enr[2];
-  __device__ void res(const fk_params &P, double x[4], double *r0, double *r1) const {
-    const int k = 1 - pm;
-    double s1 = ar[k] / x[k];
-    double s2 = ar[k] * (x[2 + k] - enr[k]);
-    x[pm] = (1 / (1 - s1)) * ar[pm];
-    x[2 + pm] = enr[pm] - (1 / ar[pm]) * s2;
-    double p0 = eos_p(P, 0, x[0], x[2]), p1 = eos_p(P, 1, x[1], x[3]);
-    double t0 = eos_T(P, 0, x[0], x[2]), t1 = eos_T(P, 1, x[1], x[3]);
-    *r0 = p0 - p1 - (double)0.0f;
-    *r1 = t0 - t1 - (double)0.0f;
-  }
-};
-
-__device__ int therm_dev(const fk_params &P, double *U) {
-  const int itrMax = P.thermItrMax;
-  const double urf = P.thermUrf;
-  double avf0 = clampd(P, U[0]);
-  double avf1 = 1 - avf0;
-  double rho0 = U[1] / avf0, rho1 = U[2] / avf1;
-  ThermSys t;
-  t.ar[0] = avf0 * rho0;
-  t.ar[1] = avf1 * rho1;
-  t.enr[0] = U[6] / t.ar[0];
-  t.enr[1] = U[7] / t.ar[1];
-  const double rhoMix = t.ar[0] + t.ar[1];
-  const double v0 = U[3] / rhoMix, v1 = U[4] / rhoMix, v2 = U[5] / rhoMix;
-  const double Y0 = t.ar[0] / rhoMix, Y1 = t.ar[1] / rhoMix;
-  double x[4] = {rho0, rho1, t.enr[0], t.enr[1]};
-  double ax[4] = {x[0], x[1], x[2], x[3]};
-  t.pm = (avf1 > avf0) ? 1 : 0;  // primary = larger volume fraction (thermRlx.cu:146-152)
-  const int pm = t.pm, k = 1 - pm;
-  const double de = U[8] / rhoMix - Y0 * t.enr[0] - Y1 * t.enr[1] - 0.5 * (v0 * v0 + v1 * v1 + v2 * v2);
-  t.enr[0] += de;
-  t.enr[1] += de;
-  int abortFlag = 0;
-  float minRes = 1e9f;
-  int itr = 0;
-  bool conv = false;
-  while (!conv) {
-    itr++;
-    double r[2];
-    t.res(P, x, &r[0], &r[1]);
-    double e[2], rd[2][2];
-    // perturb x[k] then x[k+2] (the non-primary unknowns in index order)
-    e[0] = 1e-4 * x[k];
-    x[k] += e[0];
-    t.res(P, x, &rd[0][0], &rd[1][0]);
-    x[k] -= e[0];
-    e[1] = 1e-4 * x[k + 2];
-    x[k + 2] += e[1];
-    t.res(P, x, &rd[0][1], &rd[1][1]);
-    x[k + 2] -= e[1];
-    double J[2][2];
-    for (int i = 0; i < 2; i++)
-      for (int j = 0; j < 2; j++) {
-        J[i][j] = (rd[i][j] - r[i]) / e[j];
-        if (isnan(J[i][j])) abortFlag = 1;
-      }
-    const double det = J[0][0] * J[1][1] - J[1][0] * J[0][1];
-    double I00 = J[1][1] / det, I11 = J[0][0] / det, I10 = -J[1][0] / det, I01 = -J[0][1] / det;
-    if (det == 0) abortFlag = 1;
-    if (abortFlag == 0 && x[0] > 0.0 && x[1] > 0.0) {
-      float chk = (float)sqrt(r[0] * r[0] + r[1] * r[1]);
-      if (chk < minRes) {
-        minRes = chk;
-        for (int i = 0; i < 4; ++i) ax[i] = x[i];
-      }
-    }
-    double dx0 = 0;
-    dx0 += I00 * r[0];
-    dx0 += I01 * r[1];
-    x[k] = x[k] - urf * dx0;
-    double dx1 = 0;
-    dx1 += I10 * r[0];
-    dx1 += I11 * r[1];
-    x[k + 2] = x[k + 2] - urf * dx1;
-    if (!(fabs(r[0]) > 1.e-2) && !(fabs(r[1]) > 1.e-2)) conv = true;
-    if (abortFlag == 1 || itr > itrMax) {
-      for (int i = 0; i < 4; ++i) x[i] = ax[i];
-      conv = true;
-    }
-  }
-  const double s1 = t.ar[k] / x[k];
-  const double s2 = t.ar[k] * (x[2 + k] - t.enr[k]);
-  x[pm] = (1 / (1 - s1)) * t.ar[pm];
-  x[2 + pm] = t.enr[pm] - (1 / t.ar[pm]) * s2;
-  U[0] = clampd(P, t.ar[0] / x[0]);
-  U[6] = t.ar[0] * x[2];
-  U[7] = t.ar[1] * x[3];
-  return itr;
-}
-
-__device__ __forceinline__ double gibbs_dev(const fk_params &P, int k, double pr, double T) {
-  const double et = k == 0 ? 0.0 : P.satConst[0] * log10(pr) + P.satConst[1];
-  const double h = P.Cp[k] * T + P.eta[k];
-  const double s = (P.Cp[k] / P.gamma[k]) * log10(pow(T, P.gamma[k]) / pow(pr + P.pInf[k], P.gamma[k] - 1)) + et;
-  return h - T * s;
-}
-
-struct ChemSys {
-  int pm;
-  double rhoMix, rEint;
-  __device__ void res(const fk_params &P, double x[6], double *r0, double *r1, double *r2) const {
-    const int k = 1 - pm;
-    const double s1 = x[k], s2 = x[k] * x[k + 2], s3 = x[k] * x[k + 2] * x[k + 4];
-    x[pm] = 1 - s1;
-    x[pm + 2] = (rhoMix - s2) / x[pm];
-    x[pm + 4] = (rEint - s3) / (x[pm] * x[pm + 2]);
-    double pr[2], te[2], gi[2];
-    for (int q = 0; q < 2; q++) {
-      pr[q] = eos_p(P, q, x[q + 2], x[q + 4]);
-      te[q] = eos_T(P, q, x[q + 2], x[q + 4]);
-      gi[q] = gibbs_dev(P, q, pr[q], te[q]);
-    }
-    *r0 = pr[0] - pr[1];
-    *r1 = te[0] - te[1];
-    *r2 = gi[0] - gi[1];
-  }
-};
-
-__device__ void chk_update(const fk_params &P, double xOld, double *x, double dx, int i, float urf, int *abortFlag,
-                           double rho) {
-  if (i == 0 || i == 1) {
-    if (x[i] < 0)
-      while (x[i] < P.chemLimit && urf > 0.0005) {
-        urf = urf / 2;
-        x[i] = xOld - urf * dx;
-      }
-    if (x[i] > 1)
-      while (x[i] > 1.0 - P.chemLimit && urf > 0.0005) {
-        urf = urf / 2;
-        x[i] = xOld - urf * dx;
-      }
-    if (x[i] < 0) *abortFlag = 1;
-    if (x[i] > 1) *abortFlag = 1;
-  } else if (i == 2 || i == 3) {
-    double ar = x[i - 2] * x[i];
-    if (x[i] < 0)
-      while (x[i] < 0 && urf > 0.0005) {
-        urf = urf / 2;
-        x[i] = xOld - urf * dx;
-      }
-    if (ar > rho)
-      while (ar > rho && urf > 0.0005) {
-        urf = urf / 2;
-        x[i] = xOld - urf * dx;
-        ar = x[i - 2] * x[i];
-      }
-    if (x[i] < 0) *abortFlag = 1;
-    if (ar > rho) *abortFlag = 1;
-  }
-}
-
-__device__ int chem_dev(const fk_params &P, double *U) {
-  const int itrMax = P.chemItrMax;
-  const double urf = P.chemUrf;
-  double x[6], ax[6];
-  x[0] = clampd(P, U[0]);
-  x[1] = 1 - x[0];
-  x[2] = U[1] / x[0];
-  x[3] = U[2] / x[1];
-  x[4] = U[6] / (x[0] * x[2]);
-  x[5] = U[7] / (x[1] * x[3]);
-  for (int i = 0; i < 6; ++i) ax[i] = x[i];
-  ChemSys ch;
-  ch.rhoMix = x[0] * x[2] + x[1] * x[3];
-  const double u = U[3] / ch.rhoMix, v = U[4] / ch.rhoMix, w = U[5] / ch.rhoMix;
-  ch.rEint = U[8] - 0.5 * ch.rhoMix * (u * u + v * v + w * w);
-  ch.pm = (x[1] > x[0]) ? 1 : 0;
-  const int pm = ch.pm;
-  int idx[3];  // non-primary unknowns in index order
-  {
-    int q = 0;
-    for (int j = 0; j < 6; ++j)
-      if (j != pm && j != pm + 2 && j != pm + 4) idx[q++] = j;
-  }
-  int abortFlag = 0;
-  float minRes = 1e9f;
-  int itr = 0;
-  bool conv = false;
-  while (!conv) {
-    itr++;
-    double r[3];
-    ch.res(P, x, &r[0], &r[1], &r[2]);
-    double e[3], rd[3][3];
-    for (int q = 0; q < 3; ++q) {
-      const int j = idx[q];
-      e[q] = fabs(x[j]) < 1e-9 ? 1e-10 : 1.e-5 * x[j];
-      x[j] += e[q];
-      ch.res(P, x, &rd[0][q], &rd[1][q], &rd[2][q]);
-      x[j] -= e[q];
-    }
-    double J[3][3], I[3][3];
-    for (int i = 0; i < 3; i++)
-      for (int j = 0; j < 3; j++) {
-        J[i][j] = (rd[i][j] - r[i]) / e[j];
-        if (isnan(J[i][j])) abortFlag = 1;
-      }
-    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-    if (det == 0) abortFlag = 1;
-    const double id = 1.0 / det;
-    I[0][0] = id * (J[1][1] * J[2][2] - J[1][2] * J[2][1]);
-    I[0][1] = -id * (J[0][1] * J[2][2] - J[0][2] * J[2][1]);
-    I[0][2] = id * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
-    I[1][0] = -id * (J[1][0] * J[2][2] - J[1][2] * J[2][0]);
-    I[1][1] = id * (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
-    I[1][2] = -id * (J[0][0] * J[1][2] - J[0][2] * J[1][0]);
-    I[2][0] = id * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-    I[2][1] = -id * (J[0][0] * J[2][1] - J[0][1] * J[2][0]);
-    I[2][2] = id * (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
-    if (abortFlag == 0 && x[0] > 0.0 && x[0] < 1.0 && x[1] > 0.0 && x[1] < 1.0 && x[2] > 0.0 && x[3] > 0.0) {
-      float chk = (float)sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
-      if (chk < minRes) {
-        minRes = chk;
-        for (int i = 0; i < 6; ++i) ax[i] = x[i];
-      }
-    }
-    int abortLocal = abortFlag;
-    for (int q = 0; q < 3; ++q) {
-      const int i = idx[q];
-      double dx = 0;
-      for (int j = 0; j < 3; j++) dx += I[q][j] * r[j];
-      x[i] = x[i] - urf * dx;
-      chk_update(P, x[i] + urf * dx, x, dx, i, (float)urf, &abortLocal, ch.rhoMix);
-    }
-    if (!(fabs(r[0]) > 1.e-1) && !(fabs(r[1]) > 1.e-1) && !(fabs(r[2]) > 1.e-1)) conv = true;
-    if (abortLocal == 1 || itr > itrMax) {
-      for (int i = 0; i < 6; ++i) x[i] = ax[i];
-      conv = true;
-    }
-  }
-  const int k = 1 - pm;
-  const double s1 = x[k], s2 = x[k] * x[k + 2], s3 = x[k] * x[k + 2] * x[k + 4];
-  x[pm] = 1 - s1;
-  x[pm + 2] = (ch.rhoMix - s2) / x[pm];
-  x[pm + 4] = (ch.rEint - s3) / (x[pm] * x[pm + 2]);
-  U[0] = clampd(P, x[0]);
-  U[1] = x[0] * x[2];
-  U[2] = x[1] * x[3];
-  U[6] = x[0] * x[2] * x[4];
-  U[7] = x[1] * x[3] * x[5];
-  U[9] = x[1] * x[3];
-  return itr;
-}
-
 // partitionRelaxation + thermalRlx/chemRlx in one pass: each thread
 // classifies its cell and runs the matching Newton solve in place.
 // counts: [n_chem, newton iterations, tsat iterations] (64-bit).
+using fk::clampd;
+using fk::eos_p;
+using fk::eos_T;
+using fk::tsat_dev;
+using fk::therm_dev;
+using fk::chem_dev;
+
 __global__ void __launch_bounds__(kBlock) relax_plain(fk_params P, int64_t nc, double *__restrict__ U,
                                                       unsigned long long *__restrict__ counts) {
   const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -832,6 +508,10 @@ __global__ void __launch_bounds__(kBlock) integrate(int64_t nc, const double *__
   }
 }
 
+__global__ void __launch_bounds__(kBlock) stream_copy(int64_t n16, const uint4 *__restrict__ a, uint4 *__restrict__ b) {
+  for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n16; i += (int64_t)gridDim.x * kBlock) b[i] = a[i];
+}
+
 __global__ void zero_ull(unsigned long long *p, int n) {
   int i = threadIdx.x;
   if (i < n) p[i] = 0ull;
@@ -842,66 +522,10 @@ __global__ void zero_ull(unsigned long long *p, int n) {
 // ------------------------------------------------------------------------
 
 inline unsigned grid_for(int64_t n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+// grid of the XCD-remapped kernels: a multiple of 8 blocks
+inline unsigned grid_xcd(int64_t n) { return (grid_for(n) + kXcd - 1) / kXcd * kXcd; }
 inline size_t tsz(const fk_ctx *c) { return c->p.precision == FK_MIXED ? 4 : 8; }
 inline int ng_of(const fk_ctx *c) { return 9 + ((c->p.model == FK_MODEL_SINGLE && c->p.heatFlux == 1) ? 3 : 0); }
-
-int timer_id(fk_ctx *c, const char *name) {
-  for (size_t i = 0; i < c->tm.names.size(); ++i)
-    if (c->tm.names[i] == name) return (int)i;
-  c->tm.names.emplace_back(name);
-  c->tm.counts.push_back(0);
-  c->tm.ms.push_back(0);
-  return (int)c->tm.names.size() - 1;
-}
-
-int timer_begin(fk_ctx *c, const char *name, int *slot) {
-  *slot = -1;
-  if (!c->timing) return FK_OK;
-  Timer::Pending pd;
-  pd.id = timer_id(c, name);
-  if (c->tm.pool.size() >= 2) {
-    pd.b = c->tm.pool.back();
-    c->tm.pool.pop_back();
-    pd.a = c->tm.pool.back();
-    c->tm.pool.pop_back();
-  } else {
-    HIPCHK(hipEventCreate(&pd.a));
-    HIPCHK(hipEventCreate(&pd.b));
-  }
-  HIPCHK(hipEventRecord(pd.a, c->stream));
-  c->tm.pending.push_back(pd);
-  *slot = (int)c->tm.pending.size() - 1;
-  return FK_OK;
-}
-int timer_end(fk_ctx *c, int slot) {
-  if (slot < 0) return FK_OK;
-  HIPCHK(hipEventRecord(c->tm.pending[(size_t)slot].b, c->stream));
-  return FK_OK;
-}
-int timer_flush(fk_ctx *c) {
-  for (auto &pd : c->tm.pending) {
-    HIPCHK(hipEventSynchronize(pd.b));
-    float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, pd.a, pd.b));
-    c->tm.counts[(size_t)pd.id]++;
-    c->tm.ms[(size_t)pd.id] += ms;
-    c->tm.pool.push_back(pd.a);
-    c->tm.pool.push_back(pd.b);
-  }
-  c->tm.pending.clear();
-  return FK_OK;
-}
-
-#define LAUNCH_TIMED(ctx, name, ...)                  \
-  do {                                                \
-    int slot_;                                        \
-    int rc_ = timer_begin(ctx, name, &slot_);         \
-    if (rc_) return rc_;                              \
-    __VA_ARGS__;                                      \
-    HIPCHK(hipGetLastError());                        \
-    rc_ = timer_end(ctx, slot_);                      \
-    if (rc_) return rc_;                              \
-  } while (0)
 
 int next_buffer(const fk_ctx *c) {
   for (int i = 0; i < 3; ++i)
@@ -951,7 +575,7 @@ void free_all(fk_ctx *c) {
 
 template <typename T>
 int launch_pass_g_t(fk_ctx *c) {
-  const unsigned gr = grid_for(c->nc);
+  const unsigned gr = grid_xcd(c->nc);
   const bool visc = c->p.viscFlag == 1;
   const bool heat = visc && c->p.model == FK_MODEL_SINGLE && c->p.heatFlux == 1;
   double *dg = c->debug ? c->dgrad : nullptr;
@@ -976,7 +600,7 @@ int launch_pass_g_t(fk_ctx *c) {
 
 template <typename T, int MODE>
 int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Unew) {
-  const unsigned gr = grid_for(c->nc);
+  const unsigned gr = grid_xcd(c->nc);
   const bool visc = c->p.viscFlag == 1;
   const bool heat = visc && c->p.model == FK_MODEL_SINGLE && c->p.heatFlux == 1;
   const double *U = c->U[c->ia], *U0 = c->U[c->iu0];
@@ -1044,8 +668,14 @@ int do_dt_device(fk_ctx *c) {
   return FK_OK;
 }
 
-int do_relax(fk_ctx *c, fk_relax_stats *st) {
+int do_relax(fk_ctx *c, fk_relax_stats *st, int mode = FK_RELAX_PLAIN) {
   if (c->p.model != FK_MODEL_TWO_PHASE) return fail(FK_EINVAL, "relaxation needs the two-phase model");
+  if (mode == FK_RELAX_MLP) return fk_relax_mlp_impl(c, st);
+  if (!st) {  // plain relaxation that also emits training pairs, when enabled
+    bool done = false;
+    int rc = fk_relax_emit_impl(c, &done);
+    if (rc || done) return rc;
+  }
   LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_ull, dim3(1), dim3(64), 0, c->stream, (unsigned long long *)c->lcount, 4));
   LAUNCH_TIMED(c, "relax", hipLaunchKernelGGL(relax_plain, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, c->p,
                                               c->nc, c->U[c->ia], (unsigned long long *)c->lcount));
@@ -1068,6 +698,7 @@ int check_params(const fk_params *p) {
   if (p->precision != FK_FP64 && p->precision != FK_MIXED) return fail(FK_EINVAL, "bad precision");
   if (p->rkSteps != 1 && p->rkSteps != 3) return fail(FK_EINVAL, "rkSteps must be 1 or 3");
   if (p->adaptDt == 1 && p->adaptDtItr < 1) return fail(FK_EINVAL, "adaptDtItr must be >= 1");
+  if (p->relaxMode != FK_RELAX_PLAIN && p->relaxMode != FK_RELAX_MLP) return fail(FK_EINVAL, "bad relaxMode");
   return FK_OK;
 }
 
@@ -1110,6 +741,7 @@ int fk_destroy(fk_ctx *c) {
   }
   for (auto e : c->tm.pool) (void)hipEventDestroy(e);
   free_all(c);
+  fk_mlp_free(c);
   (void)hipStreamDestroy(c->stream);
   delete c;
   return FK_OK;
@@ -1315,9 +947,9 @@ int fk_compute_rhs_update(fk_ctx *c, int irk, double dt) {
 int fk_relaxation(fk_ctx *c, int mode, fk_relax_stats *st) {
   int rc = require(c);
   if (rc) return rc;
-  if (mode != FK_RELAX_PLAIN) return fail(FK_EINVAL, "relaxation mode not available");
+  if (mode != FK_RELAX_PLAIN && mode != FK_RELAX_MLP) return fail(FK_EINVAL, "unknown relaxation mode");
   HIPCHK(hipSetDevice(c->device));
-  return do_relax(c, st);
+  return do_relax(c, st, mode);
 }
 
 int fk_cons_calc(fk_ctx *c, double *out) {
@@ -1371,7 +1003,7 @@ int fk_run_steps(fk_ctx *c, int nsteps, double *dt_last, double *phys_time) {
     else
       c->dt = p.dt;
     if (p.model == FK_MODEL_TWO_PHASE && p.relaxFlag == 1)
-      if ((rc = do_relax(c, nullptr))) return rc;
+      if ((rc = do_relax(c, nullptr, p.relaxMode))) return rc;
     c->step++;
   }
   HIPCHK(hipMemcpyAsync(h, c->dscal, sizeof h, hipMemcpyDeviceToHost, c->stream));
@@ -1434,6 +1066,39 @@ int fk_download(fk_ctx *c, const char *name, void *host, int64_t bytes) {
     return FK_OK;
   }
   return fail(FK_EINVAL, "unknown array " + s);
+}
+
+int fk_stream_copy(fk_ctx *c, int64_t bytes, double *ms) {
+  if (!c || !ms) return fail(FK_EINVAL, "null argument");
+  if (bytes < 16) return fail(FK_EINVAL, "copy needs at least 16 bytes");
+  HIPCHK(hipSetDevice(c->device));
+  const int64_t n16 = bytes / 16;
+  void *a = nullptr, *b = nullptr;
+  HIPCHK(hipMalloc(&a, (size_t)n16 * 16));
+  hipError_t e = hipMalloc(&b, (size_t)n16 * 16);
+  if (e != hipSuccess) {
+    (void)hipFree(a);
+    return fail(FK_EDEVICE, std::string("hipMalloc: ") + hipGetErrorString(e));
+  }
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipMemsetAsync(a, 1, (size_t)n16 * 16, c->stream));
+  const unsigned grid = (unsigned)std::min<int64_t>((n16 + kBlock - 1) / kBlock, 65536);
+  hipLaunchKernelGGL(stream_copy, dim3(grid), dim3(kBlock), 0, c->stream, n16, (const uint4 *)a, (uint4 *)b);  // warm
+  HIPCHK(hipEventRecord(e0, c->stream));
+  hipLaunchKernelGGL(stream_copy, dim3(grid), dim3(kBlock), 0, c->stream, n16, (const uint4 *)a, (uint4 *)b);
+  HIPCHK(hipEventRecord(e1, c->stream));
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipEventSynchronize(e1));
+  float t = 0;
+  HIPCHK(hipEventElapsedTime(&t, e0, e1));
+  *ms = t;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return FK_OK;
 }
 
 int fk_set_timing(fk_ctx *c, int on) {

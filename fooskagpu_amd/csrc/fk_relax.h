@@ -1,0 +1,460 @@
+// Device code of the per-cell phase relaxation (reference Newton solves of
+// relaxation.cu, thermRlx.cu and chemRlx.cu), shared by the plain and the
+// surrogate-warm-started relaxation kernels.  FP64 throughout.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/fooska.h"
+
+namespace fk {
+
+// ---------------- phase relaxation (relaxation.cu, thermRlx.cu, chemRlx.cu) ----------------
+
+__device__ inline double clampd(const fk_params &P, double a) {
+  return fmin(fmax(P.avfMin / 100, a), 1.0 - P.avfMin / 100);
+}
+__device__ inline double eos_p(const fk_params &P, int k, double rho, double e) {
+  return (P.gamma[k] - 1) * rho * e - P.gamma[k] * P.pInf[k] - (P.gamma[k] - 1) * P.eta[k] * rho;
+}
+__device__ inline double eos_T(const fk_params &P, int k, double rho, double e) {
+  double p = (P.gamma[k] - 1) * rho * e - P.gamma[k] * P.pInf[k] - (P.gamma[k] - 1) * P.eta[k] * rho;
+  return (p + P.pInf[k]) / ((P.Cp[k] / P.gamma[k]) * (P.gamma[k] - 1) * rho);
+}
+
+__device__ inline double tsat_dev(const fk_params &P, double Pr, double temp, double urf, int *iters) {
+  const double Pinf = P.pInf[0], g1 = P.gamma[0], g2 = P.gamma[1], Cp1 = P.Cp[0], Cp2 = P.Cp[1];
+  const double Cv1 = Cp1 / g1, Cv2 = Cp2 / g2;
+  const double et2 = P.satConst[0] * log10(Pr) + P.satConst[1];
+  const double As = (Cp1 - Cp2 + et2 - 0.0) / (Cp2 - Cv2);
+  const double Bs = (P.eta[0] - P.eta[1]) / (Cp2 - Cv2);
+  const double Cs = (Cp2 - Cp1) / (Cp2 - Cv2);
+  const double Ds = (Cp1 - Cv1) / (Cp2 - Cv2);
+  const double dt = 1.e-3;
+  double fun = 1, T = temp;
+  int itr = 0;
+  while (fabs(fun) > 1.e-2 && itr < 50) {
+    fun = As + Bs / T + Cs * log10(T) + Ds * log10(Pr + Pinf) - log10(Pr);
+    double fd = As + Bs / (T + dt) + Cs * log10(T + dt) + Ds * log10(Pr + Pinf) - log10(Pr);
+    double gr = (fd - fun) / dt;
+    T = T - urf * fun / gr;
+    itr++;
+  }
+  *iters = itr;
+  return T;
+}
+
+struct ThermSys {
+  int pm;
+  double ar[2], enr[2];
+  __device__ void res(const fk_params &P, double x[4], double *r0, double *r1) const {
+    const int k = 1 - pm;
+    double s1 = ar[k] / x[k];
+    double s2 = ar[k] * (x[2 + k] - enr[k]);
+    x[pm] = (1 / (1 - s1)) * ar[pm];
+    x[2 + pm] = enr[pm] - (1 / ar[pm]) * s2;
+    double p0 = eos_p(P, 0, x[0], x[2]), p1 = eos_p(P, 1, x[1], x[3]);
+    double t0 = eos_T(P, 0, x[0], x[2]), t1 = eos_T(P, 1, x[1], x[3]);
+    *r0 = p0 - p1 - (double)0.0f;
+    *r1 = t0 - t1 - (double)0.0f;
+  }
+};
+
+__device__ inline int therm_dev(const fk_params &P, double *U) {
+  const int itrMax = P.thermItrMax;
+  const double urf = P.thermUrf;
+  double avf0 = clampd(P, U[0]);
+  double avf1 = 1 - avf0;
+  double rho0 = U[1] / avf0, rho1 = U[2] / avf1;
+  ThermSys t;
+  t.ar[0] = avf0 * rho0;
+  t.ar[1] = avf1 * rho1;
+  t.enr[0] = U[6] / t.ar[0];
+  t.enr[1] = U[7] / t.ar[1];
+  const double rhoMix = t.ar[0] + t.ar[1];
+  const double v0 = U[3] / rhoMix, v1 = U[4] / rhoMix, v2 = U[5] / rhoMix;
+  const double Y0 = t.ar[0] / rhoMix, Y1 = t.ar[1] / rhoMix;
+  double x[4] = {rho0, rho1, t.enr[0], t.enr[1]};
+  double ax[4] = {x[0], x[1], x[2], x[3]};
+  t.pm = (avf1 > avf0) ? 1 : 0;  // primary = larger volume fraction (thermRlx.cu:146-152)
+  const int pm = t.pm, k = 1 - pm;
+  const double de = U[8] / rhoMix - Y0 * t.enr[0] - Y1 * t.enr[1] - 0.5 * (v0 * v0 + v1 * v1 + v2 * v2);
+  t.enr[0] += de;
+  t.enr[1] += de;
+  int abortFlag = 0;
+  float minRes = 1e9f;
+  int itr = 0;
+  bool conv = false;
+  while (!conv) {
+    itr++;
+    double r[2];
+    t.res(P, x, &r[0], &r[1]);
+    double e[2], rd[2][2];
+    // perturb x[k] then x[k+2] (the non-primary unknowns in index order)
+    e[0] = 1e-4 * x[k];
+    x[k] += e[0];
+    t.res(P, x, &rd[0][0], &rd[1][0]);
+    x[k] -= e[0];
+    e[1] = 1e-4 * x[k + 2];
+    x[k + 2] += e[1];
+    t.res(P, x, &rd[0][1], &rd[1][1]);
+    x[k + 2] -= e[1];
+    double J[2][2];
+    for (int i = 0; i < 2; i++)
+      for (int j = 0; j < 2; j++) {
+        J[i][j] = (rd[i][j] - r[i]) / e[j];
+        if (isnan(J[i][j])) abortFlag = 1;
+      }
+    const double det = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+    double I00 = J[1][1] / det, I11 = J[0][0] / det, I10 = -J[1][0] / det, I01 = -J[0][1] / det;
+    if (det == 0) abortFlag = 1;
+    if (abortFlag == 0 && x[0] > 0.0 && x[1] > 0.0) {
+      float chk = (float)sqrt(r[0] * r[0] + r[1] * r[1]);
+      if (chk < minRes) {
+        minRes = chk;
+        for (int i = 0; i < 4; ++i) ax[i] = x[i];
+      }
+    }
+    double dx0 = 0;
+    dx0 += I00 * r[0];
+    dx0 += I01 * r[1];
+    x[k] = x[k] - urf * dx0;
+    double dx1 = 0;
+    dx1 += I10 * r[0];
+    dx1 += I11 * r[1];
+    x[k + 2] = x[k + 2] - urf * dx1;
+    if (!(fabs(r[0]) > 1.e-2) && !(fabs(r[1]) > 1.e-2)) conv = true;
+    if (abortFlag == 1 || itr > itrMax) {
+      for (int i = 0; i < 4; ++i) x[i] = ax[i];
+      conv = true;
+    }
+  }
+  const double s1 = t.ar[k] / x[k];
+  const double s2 = t.ar[k] * (x[2 + k] - t.enr[k]);
+  x[pm] = (1 / (1 - s1)) * t.ar[pm];
+  x[2 + pm] = t.enr[pm] - (1 / t.ar[pm]) * s2;
+  U[0] = clampd(P, t.ar[0] / x[0]);
+  U[6] = t.ar[0] * x[2];
+  U[7] = t.ar[1] * x[3];
+  return itr;
+}
+
+__device__ inline double gibbs_dev(const fk_params &P, int k, double pr, double T) {
+  const double et = k == 0 ? 0.0 : P.satConst[0] * log10(pr) + P.satConst[1];
+  const double h = P.Cp[k] * T + P.eta[k];
+  const double s = (P.Cp[k] / P.gamma[k]) * log10(pow(T, P.gamma[k]) / pow(pr + P.pInf[k], P.gamma[k] - 1)) + et;
+  return h - T * s;
+}
+
+struct ChemSys {
+  int pm;
+  double rhoMix, rEint;
+  __device__ void res(const fk_params &P, double x[6], double *r0, double *r1, double *r2) const {
+    const int k = 1 - pm;
+    const double s1 = x[k], s2 = x[k] * x[k + 2], s3 = x[k] * x[k + 2] * x[k + 4];
+    x[pm] = 1 - s1;
+    x[pm + 2] = (rhoMix - s2) / x[pm];
+    x[pm + 4] = (rEint - s3) / (x[pm] * x[pm + 2]);
+    double pr[2], te[2], gi[2];
+    for (int q = 0; q < 2; q++) {
+      pr[q] = eos_p(P, q, x[q + 2], x[q + 4]);
+      te[q] = eos_T(P, q, x[q + 2], x[q + 4]);
+      gi[q] = gibbs_dev(P, q, pr[q], te[q]);
+    }
+    *r0 = pr[0] - pr[1];
+    *r1 = te[0] - te[1];
+    *r2 = gi[0] - gi[1];
+  }
+};
+
+__device__ inline void chk_update(const fk_params &P, double xOld, double *x, double dx, int i, float urf, int *abortFlag,
+                           double rho) {
+  if (i == 0 || i == 1) {
+    if (x[i] < 0)
+      while (x[i] < P.chemLimit && urf > 0.0005) {
+        urf = urf / 2;
+        x[i] = xOld - urf * dx;
+      }
+    if (x[i] > 1)
+      while (x[i] > 1.0 - P.chemLimit && urf > 0.0005) {
+        urf = urf / 2;
+        x[i] = xOld - urf * dx;
+      }
+    if (x[i] < 0) *abortFlag = 1;
+    if (x[i] > 1) *abortFlag = 1;
+  } else if (i == 2 || i == 3) {
+    double ar = x[i - 2] * x[i];
+    if (x[i] < 0)
+      while (x[i] < 0 && urf > 0.0005) {
+        urf = urf / 2;
+        x[i] = xOld - urf * dx;
+      }
+    if (ar > rho)
+      while (ar > rho && urf > 0.0005) {
+        urf = urf / 2;
+        x[i] = xOld - urf * dx;
+        ar = x[i - 2] * x[i];
+      }
+    if (x[i] < 0) *abortFlag = 1;
+    if (ar > rho) *abortFlag = 1;
+  }
+}
+
+__device__ inline int chem_dev(const fk_params &P, double *U) {
+  const int itrMax = P.chemItrMax;
+  const double urf = P.chemUrf;
+  double x[6], ax[6];
+  x[0] = clampd(P, U[0]);
+  x[1] = 1 - x[0];
+  x[2] = U[1] / x[0];
+  x[3] = U[2] / x[1];
+  x[4] = U[6] / (x[0] * x[2]);
+  x[5] = U[7] / (x[1] * x[3]);
+  for (int i = 0; i < 6; ++i) ax[i] = x[i];
+  ChemSys ch;
+  ch.rhoMix = x[0] * x[2] + x[1] * x[3];
+  const double u = U[3] / ch.rhoMix, v = U[4] / ch.rhoMix, w = U[5] / ch.rhoMix;
+  ch.rEint = U[8] - 0.5 * ch.rhoMix * (u * u + v * v + w * w);
+  ch.pm = (x[1] > x[0]) ? 1 : 0;
+  const int pm = ch.pm;
+  int idx[3];  // non-primary unknowns in index order
+  {
+    int q = 0;
+    for (int j = 0; j < 6; ++j)
+      if (j != pm && j != pm + 2 && j != pm + 4) idx[q++] = j;
+  }
+  int abortFlag = 0;
+  float minRes = 1e9f;
+  int itr = 0;
+  bool conv = false;
+  while (!conv) {
+    itr++;
+    double r[3];
+    ch.res(P, x, &r[0], &r[1], &r[2]);
+    double e[3], rd[3][3];
+    for (int q = 0; q < 3; ++q) {
+      const int j = idx[q];
+      e[q] = fabs(x[j]) < 1e-9 ? 1e-10 : 1.e-5 * x[j];
+      x[j] += e[q];
+      ch.res(P, x, &rd[0][q], &rd[1][q], &rd[2][q]);
+      x[j] -= e[q];
+    }
+    double J[3][3], I[3][3];
+    for (int i = 0; i < 3; i++)
+      for (int j = 0; j < 3; j++) {
+        J[i][j] = (rd[i][j] - r[i]) / e[j];
+        if (isnan(J[i][j])) abortFlag = 1;
+      }
+    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    if (det == 0) abortFlag = 1;
+    const double id = 1.0 / det;
+    I[0][0] = id * (J[1][1] * J[2][2] - J[1][2] * J[2][1]);
+    I[0][1] = -id * (J[0][1] * J[2][2] - J[0][2] * J[2][1]);
+    I[0][2] = id * (J[0][1] * J[1][2] - J[0][2] * J[1][1]);
+    I[1][0] = -id * (J[1][0] * J[2][2] - J[1][2] * J[2][0]);
+    I[1][1] = id * (J[0][0] * J[2][2] - J[0][2] * J[2][0]);
+    I[1][2] = -id * (J[0][0] * J[1][2] - J[0][2] * J[1][0]);
+    I[2][0] = id * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    I[2][1] = -id * (J[0][0] * J[2][1] - J[0][1] * J[2][0]);
+    I[2][2] = id * (J[0][0] * J[1][1] - J[0][1] * J[1][0]);
+    if (abortFlag == 0 && x[0] > 0.0 && x[0] < 1.0 && x[1] > 0.0 && x[1] < 1.0 && x[2] > 0.0 && x[3] > 0.0) {
+      float chk = (float)sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+      if (chk < minRes) {
+        minRes = chk;
+        for (int i = 0; i < 6; ++i) ax[i] = x[i];
+      }
+    }
+    int abortLocal = abortFlag;
+    for (int q = 0; q < 3; ++q) {
+      const int i = idx[q];
+      double dx = 0;
+      for (int j = 0; j < 3; j++) dx += I[q][j] * r[j];
+      x[i] = x[i] - urf * dx;
+      chk_update(P, x[i] + urf * dx, x, dx, i, (float)urf, &abortLocal, ch.rhoMix);
+    }
+    if (!(fabs(r[0]) > 1.e-1) && !(fabs(r[1]) > 1.e-1) && !(fabs(r[2]) > 1.e-1)) conv = true;
+    if (abortLocal == 1 || itr > itrMax) {
+      for (int i = 0; i < 6; ++i) x[i] = ax[i];
+      conv = true;
+    }
+  }
+  const int k = 1 - pm;
+  const double s1 = x[k], s2 = x[k] * x[k + 2], s3 = x[k] * x[k + 2] * x[k + 4];
+  x[pm] = 1 - s1;
+  x[pm + 2] = (ch.rhoMix - s2) / x[pm];
+  x[pm + 4] = (ch.rEint - s3) / (x[pm] * x[pm + 2]);
+  U[0] = clampd(P, x[0]);
+  U[1] = x[0] * x[2];
+  U[2] = x[1] * x[3];
+  U[6] = x[0] * x[2] * x[4];
+  U[7] = x[1] * x[3] * x[5];
+  U[9] = x[1] * x[3];
+  return itr;
+}
+
+
+// ---------------- surrogate path: exact classification + Newton polish ----------------
+// Saturation function f(T; p) of relaxation.cu:18-31.  With
+// dT f = (C T ln10 - B) / (T^2 ln10) > 0 (B <= 0, C >= 0, checked on the
+// host) f is increasing in T, so T_L > Tsat(p_L) <=> f(T_L; p_L) > 0: the
+// classification of partitionRelaxation without the Tsat iteration.
+__device__ inline double sat_fun(const fk_params &P, double Pr, double T) {
+  const double g1 = P.gamma[0], g2 = P.gamma[1], Cp1 = P.Cp[0], Cp2 = P.Cp[1];
+  const double Cv1 = Cp1 / g1, Cv2 = Cp2 / g2;
+  const double et2 = P.satConst[0] * log10(Pr) + P.satConst[1];
+  const double As = (Cp1 - Cp2 + et2) / (Cp2 - Cv2);
+  const double Bs = (P.eta[0] - P.eta[1]) / (Cp2 - Cv2);
+  const double Cs = (Cp2 - Cp1) / (Cp2 - Cv2);
+  const double Ds = (Cp1 - Cv1) / (Cp2 - Cv2);
+  return As + Bs / T + Cs * log10(T) + Ds * log10(Pr + P.pInf[0]) - log10(Pr);
+}
+
+// Thermal (p-T) equilibrium of the non-primary phase k by Newton with the
+// analytic Jacobian, from the warm start (rho_k, e_k), to |dx_i| <= tol |x_i|.
+// Same unknowns, closure and residuals as thermalRlx (thermRlx.cu:184-204).
+// Returns iterations (> 0) or -1 when it fails (the caller falls back).
+__device__ inline int therm_polish(const fk_params &P, double *U, double rk0, double ek0, double tol, int itmax) {
+  const double a0 = clampd(P, U[0]), a1 = 1 - a0;
+  const double r0 = U[1] / a0, r1 = U[2] / a1;
+  double ar[2] = {a0 * r0, a1 * r1};
+  double enr[2] = {U[6] / ar[0], U[7] / ar[1]};
+  const double rhoMix = ar[0] + ar[1];
+  const double v0 = U[3] / rhoMix, v1 = U[4] / rhoMix, v2 = U[5] / rhoMix;
+  const double de = U[8] / rhoMix - (ar[0] / rhoMix) * enr[0] - (ar[1] / rhoMix) * enr[1] - 0.5 * (v0 * v0 + v1 * v1 + v2 * v2);
+  enr[0] += de;
+  enr[1] += de;
+  const int pm = (a1 > a0) ? 1 : 0, k = 1 - pm;
+  double xr = rk0, xe = ek0;
+  if (!(xr > ar[k]) || !isfinite(xe)) return -1;
+  const double sk = k == 0 ? 1.0 : -1.0;  // sign of phase k in r = q_0 - q_1
+  for (int it = 1; it <= itmax; ++it) {
+    const double rm = ar[pm] / (1 - ar[k] / xr);
+    const double em = enr[pm] - (ar[k] / ar[pm]) * (xe - enr[k]);
+    if (!(rm > 0)) return -1;
+    const double gk = P.gamma[k], gm = P.gamma[pm];
+    const double cvk = P.Cp[k] / gk, cvm = P.Cp[pm] / gm;
+    const double pk = eos_p(P, k, xr, xe), pmm = eos_p(P, pm, rm, em);
+    const double tk = eos_T(P, k, xr, xe), tm = eos_T(P, pm, rm, em);
+    const double f0 = sk * (pk - pmm), f1 = sk * (tk - tm);
+    // d rho_m / d rho_k, d e_m / d e_k of the closure
+    const double drm = -(rm * rm / ar[pm]) * (ar[k] / (xr * xr));
+    const double dem = -ar[k] / ar[pm];
+    const double J00 = sk * ((gk - 1) * (xe - P.eta[k]) - (gm - 1) * (em - P.eta[pm]) * drm);
+    const double J01 = sk * ((gk - 1) * xr - (gm - 1) * rm * dem);
+    const double J10 = sk * (P.pInf[k] / (cvk * xr * xr) - P.pInf[pm] / (cvm * rm * rm) * drm);
+    const double J11 = sk * (1.0 / cvk - dem / cvm);
+    const double det = J00 * J11 - J01 * J10;
+    if (!(fabs(det) > 0) || !isfinite(det)) return -1;
+    const double dr = (J11 * f0 - J01 * f1) / det;
+    const double dee = (J00 * f1 - J10 * f0) / det;
+    xr -= dr;
+    xe -= dee;
+    if (!(xr > ar[k]) || !isfinite(xe)) return -1;
+    if (fabs(dr) <= tol * fabs(xr) && fabs(dee) <= tol * fabs(xe)) {
+      const double s1 = ar[k] / xr, s2 = ar[k] * (xe - enr[k]);
+      double x[4];
+      x[k] = xr;
+      x[2 + k] = xe;
+      x[pm] = (1 / (1 - s1)) * ar[pm];
+      x[2 + pm] = enr[pm] - (1 / ar[pm]) * s2;
+      U[0] = clampd(P, ar[0] / x[0]);
+      U[6] = ar[0] * x[2];
+      U[7] = ar[1] * x[3];
+      return it;
+    }
+  }
+  return -1;
+}
+
+// Chemical (p-T-g) equilibrium of chemRlx (chemRlx.cu:52-76, 195-215) by
+// Newton with a forward-difference Jacobian (step 1e-7 |x|) and full steps,
+// from the warm start (alpha_k, rho_k, e_k), to |dx_i| <= tol |x_i|.
+// Returns iterations or -1 (out of bounds, singular, not converged).
+__device__ inline int chem_polish(const fk_params &P, double *U, double ak0, double rk0, double ek0, double tol,
+                                  int itmax) {
+  double x0[6];
+  x0[0] = clampd(P, U[0]);
+  x0[1] = 1 - x0[0];
+  ChemSys ch;
+  ch.rhoMix = x0[0] * (U[1] / x0[0]) + x0[1] * (U[2] / x0[1]);
+  const double u = U[3] / ch.rhoMix, v = U[4] / ch.rhoMix, w = U[5] / ch.rhoMix;
+  ch.rEint = U[8] - 0.5 * ch.rhoMix * (u * u + v * v + w * w);
+  ch.pm = (x0[1] > x0[0]) ? 1 : 0;
+  const int pm = ch.pm, k = 1 - pm;
+  double x[6] = {0, 0, 0, 0, 0, 0};
+  x[k] = ak0;
+  x[k + 2] = rk0;
+  x[k + 4] = ek0;
+  for (int it = 1; it <= itmax; ++it) {
+    double r[3], rd[3][3], e[3];
+    ch.res(P, x, &r[0], &r[1], &r[2]);
+    const int idx[3] = {k, k + 2, k + 4};
+    for (int q = 0; q < 3; ++q) {
+      const int j = idx[q];
+      e[q] = 1e-7 * fabs(x[j]) + 1e-30;
+      const double keep = x[j];
+      x[j] = keep + e[q];
+      ch.res(P, x, &rd[0][q], &rd[1][q], &rd[2][q]);
+      x[j] = keep;
+    }
+    double J[3][3];
+    for (int i = 0; i < 3; ++i)
+      for (int j = 0; j < 3; ++j) J[i][j] = (rd[i][j] - r[i]) / e[j];
+    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
+                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    if (!(fabs(det) > 0) || !isfinite(det)) return -1;
+    // Cramer's rule for J dx = r
+    double dx[3];
+    for (int c = 0; c < 3; ++c) {
+      double M[3][3];
+      for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) M[i][j] = j == c ? r[i] : J[i][j];
+      dx[c] = (M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
+               M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0])) / det;
+    }
+    bool small = true;
+    for (int q = 0; q < 3; ++q) {
+      const int j = idx[q];
+      x[j] -= dx[q];
+      if (!(fabs(dx[q]) <= tol * fabs(x[j]))) small = false;
+    }
+    // bounds of checkUpdate (chemRlx.cu:139-162)
+    if (!(x[k] > 0 && x[k] < 1 && x[k + 2] > 0 && x[k] * x[k + 2] <= ch.rhoMix) || !isfinite(x[k + 4])) return -1;
+    if (small) {
+      const double s1 = x[k], s2 = x[k] * x[k + 2], s3 = x[k] * x[k + 2] * x[k + 4];
+      x[pm] = 1 - s1;
+      x[pm + 2] = (ch.rhoMix - s2) / x[pm];
+      x[pm + 4] = (ch.rEint - s3) / (x[pm] * x[pm + 2]);
+      if (!(x[pm + 2] > 0)) return -1;
+      U[0] = clampd(P, x[0]);
+      U[1] = x[0] * x[2];
+      U[2] = x[1] * x[3];
+      U[6] = x[0] * x[2] * x[4];
+      U[7] = x[1] * x[3] * x[5];
+      U[9] = x[1] * x[3];
+      return it;
+    }
+  }
+  return -1;
+}
+
+// Surrogate features of a conserved state: (log rho, e, alpha_L) with e the
+// mixture internal energy per unit mass.
+__device__ inline void mlp_features(const fk_params &P, const double *U, float *x) {
+  const double rho = U[1] + U[2];
+  const double ke = 0.5 * (U[3] * U[3] + U[4] * U[4] + U[5] * U[5]) / rho;
+  x[0] = (float)log(rho);
+  x[1] = (float)((U[8] - ke) / rho);
+  x[2] = (float)clampd(P, U[0]);
+}
+
+// Targets of a relaxed state: (T_L, asinh(p_L / 1e4), alpha_L).
+__device__ inline void mlp_targets(const fk_params &P, const double *U, float *y) {
+  const double a = clampd(P, U[0]);
+  const double rl = U[1] / a, el = U[6] / (a * rl);
+  y[0] = (float)eos_T(P, 0, rl, el);
+  y[1] = (float)asinh(eos_p(P, 0, rl, el) / 1e4);
+  y[2] = (float)a;
+}
+
+}  // namespace fk

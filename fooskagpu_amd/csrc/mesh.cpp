@@ -206,6 +206,45 @@ std::vector<int64_t> rcm_order(const fkm_mesh &m) {
   return perm;
 }
 
+// Morton (Z-order) of the cell centroids quantised to 21 bits per axis over
+// the bounding box; ties keep the previous order.  Returns perm[new] = old.
+std::vector<int64_t> morton_order(const fkm_mesh &m) {
+  const int64_t nc = m.nc;
+  double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+  for (int64_t i = 0; i < m.nn; ++i)
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = std::min(lo[a], m.nodes[(size_t)(i * 3 + a)]);
+      hi[a] = std::max(hi[a], m.nodes[(size_t)(i * 3 + a)]);
+    }
+  auto spread = [](uint64_t v) {  // 21 bits -> every third bit
+    v &= 0x1fffffULL;
+    v = (v | v << 32) & 0x1f00000000ffffULL;
+    v = (v | v << 16) & 0x1f0000ff0000ffULL;
+    v = (v | v << 8) & 0x100f00f00f00f00fULL;
+    v = (v | v << 4) & 0x10c30c30c30c30c3ULL;
+    v = (v | v << 2) & 0x1249249249249249ULL;
+    return v;
+  };
+  std::vector<std::pair<uint64_t, int64_t>> key((size_t)nc);
+  for (int64_t c = 0; c < nc; ++c) {
+    uint64_t code = 0;
+    for (int a = 0; a < 3; ++a) {
+      double x = 0;
+      for (int q = 0; q < 4; ++q) x += m.nodes[(size_t)(m.el2nd[(size_t)(c * 4 + q)] * 3 + a)];
+      x *= 0.25;
+      const double span = hi[a] > lo[a] ? hi[a] - lo[a] : 1.0;
+      double f = (x - lo[a]) / span;
+      f = std::min(std::max(f, 0.0), 1.0);
+      code |= spread((uint64_t)(f * 2097151.0)) << a;
+    }
+    key[(size_t)c] = {code, c};
+  }
+  std::sort(key.begin(), key.end());
+  std::vector<int64_t> perm((size_t)nc);
+  for (int64_t i = 0; i < nc; ++i) perm[(size_t)i] = key[(size_t)i].second;
+  return perm;
+}
+
 void apply_order(fkm_mesh &m, const std::vector<int64_t> &perm) {
   const int64_t nc = m.nc;
   std::vector<int64_t> inv((size_t)nc);
@@ -366,6 +405,7 @@ int finish_mesh(fkm_mesh &m, int order) {
       if (m.nb[(size_t)(c * 4 + j)] < 0 && m.elbc[(size_t)(c * 4 + j)] == 0)
         m.elbc[(size_t)(c * 4 + j)] = 2;  // untagged boundary faces: zero-gradient outlet
   if (order == 1) apply_order(m, rcm_order(m));
+  else if (order == 2) apply_order(m, morton_order(m));
   return build_faces_and_geometry(m);
 }
 
@@ -380,7 +420,7 @@ int fkm_box(int nx, int ny, int nz, const double lo[3], const double hi[3], doub
   if (!out || !lo || !hi || !bc) return fail(FK_EINVAL, "null argument");
   *out = nullptr;
   if (nx < 1 || ny < 1 || nz < 1) return fail(FK_EINVAL, "box needs at least one hex per axis");
-  if (order != 0 && order != 1) return fail(FK_EINVAL, "order must be 0 or 1");
+  if (order < 0 || order > 2) return fail(FK_EINVAL, "order must be 0, 1 or 2");
   if (jitter < 0 || jitter > 0.2) return fail(FK_EINVAL, "jitter must lie in [0, 0.2]");
   const int N[3] = {nx, ny, nz};
   for (int a = 0; a < 3; ++a) {

@@ -65,17 +65,21 @@ class Mesh:
             pass
 
     # ---- construction ----
+    ORDERS = {"none": 0, "rcm": 1, "morton": 2}
+
     @classmethod
     def box(cls, nx, ny, nz, lo=(0, 0, 0), hi=(1, 1, 1), jitter=0.0, seed=0, periodic=(False, False, False),
-            bc=(BC_OUTLET,) * 6, rcm=True):
+            bc=(BC_OUTLET,) * 6, rcm=True, order=None):
+        """order: "rcm" (default), "morton" or "none" (rcm=False)."""
         L = _lib.mesh_lib()
         lo_a = np.ascontiguousarray(lo, dtype=np.float64)
         hi_a = np.ascontiguousarray(hi, dtype=np.float64)
         bc_a = np.ascontiguousarray(bc, dtype=np.int32)
         mask = sum(1 << i for i, p in enumerate(periodic) if p)
         h = ctypes.c_void_p()
+        o = cls.ORDERS[order] if order is not None else (1 if rcm else 0)
         _lib.check_mesh(L.fkm_box(nx, ny, nz, _lib.dptr(lo_a), _lib.dptr(hi_a), jitter, seed, mask,
-                                  _lib.iptr(bc_a), 1 if rcm else 0, ctypes.byref(h)))
+                                  _lib.iptr(bc_a), o, ctypes.byref(h)))
         return cls(h)
 
     @classmethod

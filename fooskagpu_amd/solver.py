@@ -124,6 +124,12 @@ class GpuSolver:
                                                   out.nbytes))
         return out
 
+    def stream_copy(self, nbytes):
+        """Duration (ms) of a plain device copy of nbytes (HBM reference)."""
+        ms = ctypes.c_double()
+        _lib.check_hip(_lib.hip_lib().fk_stream_copy(self._h, nbytes, ctypes.byref(ms)))
+        return ms.value
+
     def set_timing(self, on=True):
         _lib.check_hip(_lib.hip_lib().fk_set_timing(self._h, 1 if on else 0))
 
@@ -151,5 +157,5 @@ class GpuSolver:
                     dt = self.compute_time_step()
                 self.update(irk, dt)
             if p.model == _lib.MODEL_TWO_PHASE and p.relaxFlag == 1:
-                self.relaxation()
+                self.relaxation(p.relaxMode)
         return dt

@@ -57,7 +57,7 @@ typedef struct fk_params {
   int32_t thermItrMax, chemItrMax;
   int32_t hllcEnergy; /* 0: energy star state as in starRegion.h:32, 1: Toro's form */
   int32_t relaxFlag;  /* two-phase: run the per-step phase relaxation */
-  int32_t pad_;
+  int32_t relaxMode;  /* FK_RELAX_PLAIN or FK_RELAX_MLP for the relaxation of fk_run_steps */
   double dt;          /* fixed time step (adaptDt == 0) */
   double cfl, maxDt, avfMin;
   double thermUrf, chemUrf, chemLimit, satTempUrf;
@@ -136,6 +136,41 @@ int fk_sync(fk_ctx *ctx);
  *  "signal" [nElem]           max signal speed per cell                      */
 int fk_download(fk_ctx *ctx, const char *name, void *host, int64_t bytes);
 int fk_set_debug(fk_ctx *ctx, int on);
+
+/* ---- surrogate of the relaxation Newton solve (no counterpart in the
+ * reference; north-star learned part).  A 3-64-64-64-3 tanh MLP maps the
+ * cell features (log rho, e, alpha_L) to the equilibrium (T, asinh(p/1e4),
+ * alpha_L); FK_RELAX_MLP uses it as the warm start of an FP64 Newton polish
+ * and falls back to the plain solve where the polish fails.  Parameters are
+ * FK_MLP_PARAMS floats in the order W1[64][3] b1 W2[64][64] b2 W3[64][64] b3
+ * W4[3][64] b4; norm[12] = feature mean[3], std[3], target mean[3], std[3]. */
+#define FK_MLP_PARAMS 8771
+int fk_mlp_set_weights(fk_ctx *ctx, const float *w, const float *norm);
+int fk_mlp_get_weights(fk_ctx *ctx, float *w, float *norm);
+int fk_mlp_set_norm(fk_ctx *ctx, const float *norm);
+/* Features and plain-relaxation targets of host states U[n][10] (the
+ * solver state is untouched). */
+int fk_mlp_label(fk_ctx *ctx, const double *U, int64_t n, float *x, float *y);
+/* Training set on the device: replace (append = 0) or extend it. */
+int fk_mlp_data(fk_ctx *ctx, const float *x, const float *y, int64_t n, int append);
+/* From now on every plain relaxation also adds the (features, targets) of
+ * cells phase + k * stride (phase advancing per call) to the training set,
+ * up to `capacity` more samples; stride 0 stops emitting. */
+int fk_mlp_emit(fk_ctx *ctx, int64_t stride, int64_t capacity);
+int fk_mlp_download_data(fk_ctx *ctx, float *x, float *y, int64_t max, int64_t *n);
+/* Gradient of the mean squared error (normalised targets) over the samples
+ * idx[B] of the training set, without an update (gradient checks). */
+int fk_mlp_grad(fk_ctx *ctx, const int64_t *idx, int64_t B, float *grad, float *loss);
+/* `steps` Adam steps on random batches (splitmix64(seed)) of the training set. */
+int fk_mlp_train(fk_ctx *ctx, int steps, int batch, float lr, uint64_t seed, float *loss_first, float *loss_last);
+/* Forward pass on raw features x[n][3] -> raw targets y[n][3]. */
+int fk_mlp_infer(fk_ctx *ctx, const float *x, int64_t n, float *y);
+
+/* Plain device-to-device copy of `bytes` (rounded down to 16 B) between two
+ * scratch buffers by a streaming kernel ("stream_copy"); returns its duration.
+ * Used to calibrate HBM byte counters and as the achievable-bandwidth
+ * reference.  No counterpart in the reference. */
+int fk_stream_copy(fk_ctx *ctx, int64_t bytes, double *ms);
 
 /* Per-kernel timing with HIP events on the context's stream.  When on, every
  * launch is bracketed by events; fk_kernel_times reports, for each named

@@ -18,7 +18,8 @@
  *   faceConnectivity    mulitphase Solver/doublePrecision/buildLocalMesh.cu:244
  *   meshProperties      mulitphase Solver/doublePrecision/meshProperties.cu:139
  *   initialize/initVec  mulitphase Solver/doublePrecision/initialize.cu:18,61
- * Cells are renumbered in reverse Cuthill-McKee order before faces are built.
+ * Cells may be renumbered (reverse Cuthill-McKee or Morton order of the
+ * centroids, see fkm_box) before faces are built.
  *
  * Every function returns 0 on success and a negative code on error;
  * fkm_last_error() gives the message of the last error on the calling thread.
@@ -49,12 +50,14 @@ const char *fkm_last_error(void);
  * are moved by a uniform random offset in [-jitter, +jitter] * spacing per
  * coordinate, drawn from splitmix64(seed).  periodic_mask bit 0/1/2 = x/y/z.
  * bc[6] = boundary code of the faces at x0, x1, y0, y1, z0, z1.
- * order: 0 = generation order (hex-major), 1 = reverse Cuthill-McKee. */
+ * order: 0 = generation order (hex-major), 1 = reverse Cuthill-McKee,
+ * 2 = Morton (Z-order) of the cell centroids. */
 int fkm_box(int nx, int ny, int nz, const double lo[3], const double hi[3],
             double jitter, uint64_t seed, int periodic_mask, const int bc[6],
             int order, fkm_mesh **out);
 
-/* Gambit neutral file (tets only), node coordinates multiplied by scale.
+/* Gambit neutral file (tets only), node coordinates multiplied by scale;
+ * order as for fkm_box.
  * BC tags 45/26/13/42/51 map to codes 1/2/3/4/5 as in translator.cu:299-349. */
 int fkm_read_neu(const char *path, double scale, int order, fkm_mesh **out);
 /* Writes the mesh (generation-order-independent) as a Gambit neutral file. */

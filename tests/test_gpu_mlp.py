@@ -1,0 +1,112 @@
+"""GPU tests of the relaxation surrogate: BF16-MFMA forward and backward
+against the CPU reference network, training, and the warm-started FP64
+Newton polish against the plain reference relaxation."""
+import numpy as np
+import pytest
+
+from fooskagpu_amd import GpuSolver, _lib, cases, surrogate
+from oracle import mlp
+
+pytestmark = pytest.mark.gpu
+
+
+def small_solver(n=10):
+    c = cases.bubble(n=n)
+    return c, GpuSolver(c.params, c.mesh, c.U)
+
+
+def random_setup(sg, n=1000, seed=0):
+    rng = np.random.default_rng(seed)
+    x = rng.normal(size=(n, 3)).astype(np.float32) * [2.0, 3e5, 0.3] + [5.0, 1.4e6, 0.5]
+    y = rng.normal(size=(n, 3)).astype(np.float32) * [20.0, 2.0, 0.3] + [300.0, 5.0, 0.5]
+    norm = surrogate.fit_norm(x, y)
+    w = surrogate.init_weights(seed + 1)
+    w[4352:4416] = rng.normal(scale=0.1, size=64)  # non-zero b2
+    sg.set_weights(w, norm)
+    sg.set_data(x, y)
+    return x, y, w, norm
+
+
+def test_forward_matches_cpu_reference(gpu):
+    c, g = small_solver()
+    with g:
+        sg = surrogate.Surrogate(g)
+        x, _, w, norm = random_setup(sg)
+        yg = sg.infer(x)
+    xn = (x - norm[0:3]) / norm[3:6]
+    yn, _ = mlp.forward(w, xn, bf16_round=True)
+    yc = yn * norm[9:12] + norm[6:9]
+    err = np.abs(yg - yc) / norm[9:12]
+    print(f"forward: max |dy|/std {err.max():.2e}, mean {err.mean():.2e}")
+    assert err.max() < 2e-2 and err.mean() < 2e-3
+
+
+def test_backward_matches_cpu_reference(gpu):
+    c, g = small_solver()
+    with g:
+        sg = surrogate.Surrogate(g)
+        x, y, w, norm = random_setup(sg, n=300, seed=4)
+        idx = np.arange(0, 300, 1)
+        gg, loss = sg.grad(idx)
+    xn = (x - norm[0:3]) / norm[3:6]
+    tn = (y - norm[6:9]) / norm[9:12]
+    gc = mlp.backward(w, xn[idx], tn[idx], bf16_round=True)
+    lc = mlp.loss(w, xn[idx], tn[idx], bf16_round=True)
+    rel = np.linalg.norm(gg - gc) / np.linalg.norm(gc)
+    cos = gg @ gc / (np.linalg.norm(gg) * np.linalg.norm(gc))
+    print(f"backward: relative L2 {rel:.2e}, cosine {cos:.6f}, loss gpu {loss:.6f} cpu {lc:.6f}")
+    assert rel < 2e-2 and cos > 0.9995
+    assert loss == pytest.approx(lc, rel=1e-3)
+    for name, (a, b) in {"W1": (0, 192), "W2": (256, 4352), "W3": (4416, 8512), "W4": (8576, 8768)}.items():
+        r = np.linalg.norm(gg[a:b] - gc[a:b]) / np.linalg.norm(gc[a:b])
+        assert r < 3e-2, (name, r)
+
+
+def test_training_reduces_loss_and_is_deterministic(gpu):
+    out = []
+    for _ in range(2):
+        c, g = small_solver()
+        with g:
+            sg = surrogate.Surrogate(g)
+            U = surrogate.uniform_states(20000, c.params, seed=2)
+            x, y = sg.label(U)
+            norm = surrogate.fit_norm(x, y)
+            sg.set_data(x, y)
+            sg.set_weights(surrogate.init_weights(0), norm)
+            l0, l1 = sg.train(300, batch=2048, lr=2e-3, seed=7)
+            out.append((l0, l1, sg.weights()[0]))
+    print(f"loss {out[0][0]:.4f} -> {out[0][1]:.5f}")
+    assert out[0][1] < 0.1 * out[0][0]
+    assert np.array_equal(out[0][2], out[1][2])
+
+
+def test_warm_started_polish_agrees_with_plain_relaxation(gpu):
+    """The surrogate changes the Newton iterations, not the converged
+    equilibrium: MLP warm start + FP64 polish vs the plain reference solve on
+    solver states of the bubble case (with a chem region)."""
+    from test_gpu_parity import cavitation
+    c = cavitation(10)
+    p = dict(c.params, relaxFlag=0)
+    with GpuSolver(p, c.mesh, c.U) as g:
+        g.run_steps(2)
+        U = g.download_state()
+        sg = surrogate.Surrogate(g)
+        xs, ys = sg.label(U)
+        xu, yu = sg.label(surrogate.uniform_states(50000, c.params, seed=3))
+        x, y = np.concatenate([xs, xu]), np.concatenate([ys, yu])
+        sg.set_data(x, y)
+        sg.set_weights(surrogate.init_weights(0), surrogate.fit_norm(x, y))
+        sg.train(600, batch=2048, lr=2e-3, seed=1)
+        g.upload_state(U)
+        st_plain = g.relaxation(_lib.RELAX_PLAIN)
+        Up = g.download_state()
+        g.upload_state(U)
+        st_mlp = g.relaxation(_lib.RELAX_MLP)
+        Um = g.download_state()
+    print(f"plain {st_plain}\nmlp   {st_mlp}")
+    err = np.abs(Um - Up).sum(0) / np.maximum(np.abs(Up).sum(0), 1e-300)
+    print("rel L1 per variable", err)
+    assert st_mlp["n_chem"] == st_plain["n_chem"] and st_plain["n_chem"] > 0
+    assert st_mlp["newton_iters"] < st_plain["newton_iters"]
+    assert st_mlp["n_fallback"] < 0.05 * len(U)
+    assert err.max() < 1e-6
