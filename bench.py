@@ -48,12 +48,27 @@ def relax_bytes(neq):
     return 8 * neq + 48
 
 
-def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None):
+def kernel_bytes(name, B, neq):
+    """Algorithmic bytes per cell of one launch of kernel `name` (None when
+    the kernel has no per-cell byte model)."""
+    return {
+        "pass_g": B["g"],
+        "pass_f": B["f_u0"],          # fused stage update (stages 2..s read U0)
+        "pass_f_res": B["f_nou0"],    # residual written in place of U_new
+        "pass_f_rhs": B["f_nou0"] + 8,  # + per-cell time-step bound
+        "rk_update": 8 * neq * 4 + 8,  # U, U0, residual read, U_new written, V
+        "relax": relax_bytes(neq),
+        "relax_screen": 8 * 9 + 8 * 3,  # U[0..8] read, alpha, alpha_k rho_k e_k written
+    }.get(name)
+
+
+def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None, fuse=1):
     """Time `steps` explicitLoop steps on the device (state resident in HBM).
     train: dict of surrogate-training options (done before the timed region)."""
     from fooskagpu_amd import GpuSolver, surrogate
     p = dict(case.params, precision=precision, relaxMode=0)
     g = GpuSolver(p, case.mesh, case.U, device=device)
+    g.set_tuning("fuse_update", fuse)
     info = {}
     if train is not None:
         t0 = time.perf_counter()
@@ -67,10 +82,20 @@ def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None):
     g.upload_state(case.U)
     g.sync()
     g.set_timing(True)
+    if case.neq == 10:
+        g.relax_totals(reset=True)
     t0 = time.perf_counter()
     dt, tphys = g.run_steps(steps)  # returns after a stream sync
     el = time.perf_counter() - t0
     kt = g.kernel_times()
+    if case.neq == 10:
+        rt = g.relax_totals()
+        cs = max(1, case.mesh.n_cells * steps)
+        info["relaxation"] = {"newton_iters_per_cell": rt["newton_iters"] / cs,
+                              "tsat_iters_per_cell": rt["tsat_iters"] / cs,
+                              "chem_fraction": rt["n_chem"] / cs, "warm_fraction": rt["n_warm"] / cs,
+                              "fallback_fraction": rt["n_fallback"] / cs}
+        log(f"relaxation: {info['relaxation']}")
     U = g.download_state()
     g.close()
     return el, kt, U, dt, info
@@ -107,6 +132,8 @@ def main():
     ap.add_argument("--no-mlp", action="store_true", help="skip the surrogate run")
     ap.add_argument("--order", default="morton", choices=["rcm", "morton", "none"], help="cell order of the mesh")
     ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--fuse", type=int, default=1, choices=[0, 1],
+                    help="RK update inside pass F (1) or residual + separate update kernel (0); same results")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30,
                     help="size of the plain device copy measured as the HBM reference (0: skip)")
     args = ap.parse_args()
@@ -139,7 +166,7 @@ def main():
         if (name == "fp64" and args.no_fp64) or (name == "mixed_mlp" and args.no_mlp):
             continue
         log(f"{name}: warmup {args.warmup}, timing {args.steps} steps")
-        el, kt, U, dt, info = run_gpu(case, prec, args.steps, args.warmup, relax_mode=mode, train=tr)
+        el, kt, U, dt, info = run_gpu(case, prec, args.steps, args.warmup, relax_mode=mode, train=tr, fuse=args.fuse)
         rate = C * s * args.steps / el
         log(f"{name}: {el:.3f} s, {rate:.4g} cell-stage/s, last dt {dt:.3e}, finite={bool(np.isfinite(U).all())}")
         for k, (n, ms) in sorted(kt.items(), key=lambda kv: -kv[1][1]):
@@ -151,9 +178,12 @@ def main():
     sb = 4
     B = stage_bytes(10, s, fc, sb, sb)
     kt = mixed["kt"]
-    n_f, ms_f = kt.get("pass_f", (0, 0.0))
+    # dominant kernel (largest total time) with a byte model
+    dom = max((k for k in kt if kernel_bytes(k, B, 10)), key=lambda k: kt[k][1])
+    n_f, ms_f = kt[dom]
     avg_f = ms_f / max(n_f, 1) / 1e3
-    achieved = B["f_u0"] * C / avg_f if avg_f > 0 else 0.0
+    bpc = kernel_bytes(dom, B, 10)
+    achieved = bpc * C / avg_f if avg_f > 0 else 0.0
     whole = (B["total"] * C * s * args.steps + relax_bytes(10) * C * args.steps) / mixed["el"]
     out = {
         "metric": "cell-RK-stage updates/s, 1 MI355X: mixed+surrogate vs FP64; % of HBM peak",
@@ -172,15 +202,16 @@ def main():
                                f"WALE, SSP-RK3, CFL 0.1, mixed precision, "
                                + ("MLP warm start + FP64 Newton polish" if head == "mixed_mlp" else
                                   "plain Newton relaxation"),
-                   "cells": C, "faces": case.mesh.n_faces, "cell_order": args.order},
+                   "cells": C, "faces": case.mesh.n_faces, "cell_order": args.order, "fuse_update": args.fuse},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
-                     "kernel": "pass_f", "bytes_per_cell": B["f_u0"]},
+                     "kernel": dom, "bytes_per_cell": bpc, "avg_launch_ms": avg_f * 1e3},
         "whole_step_hbm_frac": whole / HBM_PEAK,
         "stream_copy": copy,
     }
     if head == "mixed_mlp":
-        out["surrogate_training"] = mixed["info"]
+        out["surrogate_training"] = {k: v for k, v in mixed["info"].items() if k != "relaxation"}
+    out["relaxation"] = {k: r["info"].get("relaxation") for k, r in res.items()}
     for other in ("mixed",):
         if other in res and other != head:
             out[other] = {"value": res[other]["rate"], "ms_per_step": res[other]["el"] / args.steps * 1e3}

@@ -18,6 +18,7 @@ FP64 = 0
 MIXED = 1
 RELAX_PLAIN = 0
 RELAX_MLP = 1
+RELAX_COLD = 2
 
 
 class FkParams(ctypes.Structure):
@@ -51,7 +52,8 @@ class FkParams(ctypes.Structure):
 
 
 class FkRelaxStats(ctypes.Structure):
-    _fields_ = [(n, ctypes.c_int64) for n in ("n_therm", "n_chem", "newton_iters", "tsat_iters", "n_fallback")]
+    _fields_ = [(n, ctypes.c_int64) for n in ("n_therm", "n_chem", "newton_iters", "tsat_iters", "n_fallback",
+                                               "n_warm")]
 
 
 class FkmPhaseEos(ctypes.Structure):
@@ -110,11 +112,13 @@ HIP_API = [
     ("fk_update", ctypes.c_int, [_ctx, ctypes.c_int, ctypes.c_double]),
     ("fk_compute_rhs_update", ctypes.c_int, [_ctx, ctypes.c_int, ctypes.c_double]),
     ("fk_relaxation", ctypes.c_int, [_ctx, ctypes.c_int, ctypes.POINTER(FkRelaxStats)]),
+    ("fk_relax_totals", ctypes.c_int, [_ctx, ctypes.POINTER(FkRelaxStats), ctypes.c_int]),
     ("fk_cons_calc", ctypes.c_int, [_ctx, c_double_p]),
     ("fk_run_steps", ctypes.c_int, [_ctx, ctypes.c_int, c_double_p, c_double_p]),
     ("fk_sync", ctypes.c_int, [_ctx]),
     ("fk_download", ctypes.c_int, [_ctx, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64]),
     ("fk_set_debug", ctypes.c_int, [_ctx, ctypes.c_int]),
+    ("fk_set_tuning", ctypes.c_int, [_ctx, ctypes.c_char_p, ctypes.c_int64]),
     ("fk_stream_copy", ctypes.c_int, [_ctx, ctypes.c_int64, c_double_p]),
     ("fk_mlp_set_weights", ctypes.c_int, [_ctx, ctypes.c_void_p, ctypes.c_void_p]),
     ("fk_mlp_get_weights", ctypes.c_int, [_ctx, ctypes.c_void_p, ctypes.c_void_p]),

@@ -52,6 +52,9 @@ struct Mlp {
   int64_t n_data = 0, cap_data = 0;
   // emission of solver states during plain relaxations
   int64_t emit_stride = 0, emit_phase = 0;
+  // work list of the surrogate relaxation: (cell << 1) | chem
+  uint32_t *list = nullptr;
+  int64_t list_cap = 0;
 };
 
 }  // namespace fkx
@@ -74,10 +77,15 @@ struct fk_ctx {
   void *face = nullptr, *gvel = nullptr;
   double *rhs = nullptr, *dtc = nullptr, *red = nullptr, *dscal = nullptr;
   double *dgrad = nullptr, *dtheta = nullptr;
+  // relaxation counters: [0, 5) of the last relaxation (n_chem, Newton
+  // iterations, Tsat iterations, fallbacks, warm-started cells), [8, 13)
+  // summed over the relaxations of fk_run_steps
   long long *lcount = nullptr;
+  int64_t relax_calls = 0;  // relaxations summed in lcount[8, 13)
   double dt = 0, time = 0;
   int64_t step = 0;
   bool rhs_valid = false;
+  bool fuse_update = true;  // RK update inside pass F (else residual + rk_update)
   fkx::Timer tm;
   fkx::Mlp mlp;
   // host geometry kept to rebuild the precision-dependent arrays
@@ -85,7 +93,7 @@ struct fk_ctx {
 };
 
 // defined in fk_mlp.hip
-int fk_relax_mlp_impl(fk_ctx *c, fk_relax_stats *st);
+int fk_relax_mlp_impl(fk_ctx *c, bool warm);
 int fk_relax_emit_impl(fk_ctx *c, bool *done);
 void fk_mlp_free(fk_ctx *c);
 

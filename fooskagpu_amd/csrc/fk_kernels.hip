@@ -189,7 +189,8 @@ __global__ void __launch_bounds__(kBlock) pass_g(fk_params P, int64_t nc, const 
 
 // Pass F: Riemann problem on each face of the cell, non-conservative terms,
 // pressure correction, viscous/WALE terms, per-cell gather and (MODE 0) the
-// RK stage, or (MODE 1) residual + signal speed for the time step.
+// RK stage, (MODE 1) residual + signal speed for the time step, or (MODE 2)
+// the residual only.
 // (computeRHS.cu:78-146, computeViscousStresses.cu:66-174, update.cu:3-103,
 // computeTimeStep.cu:97-159)
 template <int NEQ, typename T, bool VISC, bool HEAT, int MODE>
@@ -363,10 +364,10 @@ __global__ void __launch_bounds__(kBlock) pass_f(fk_params P, int64_t nc, const 
       if (!(sig >= cm)) sig = cm;
     }
   }
-  if constexpr (MODE == 1) {
+  if constexpr (MODE != 0) {
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) rhs_out[c * NEQ + v] = rhs[v];
-    dtc[c] = volcfl[c] / (double)sig;
+    if constexpr (MODE == 1) dtc[c] = volcfl[c] / (double)sig;
   } else {
     const double dt = dt_ptr ? *dt_ptr : dt_val;
     const double Vc = vol[c];
@@ -517,6 +518,12 @@ __global__ void zero_ull(unsigned long long *p, int n) {
   if (i < n) p[i] = 0ull;
 }
 
+// relaxation totals: counts[8 + i] += counts[i], i < 5
+__global__ void accum_counts(unsigned long long *p) {
+  int i = threadIdx.x;
+  if (i < 5) p[8 + i] += p[i];
+}
+
 // ------------------------------------------------------------------------
 // host helpers
 // ------------------------------------------------------------------------
@@ -607,7 +614,7 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
   const T *face = (const T *)c->face;
   const T *gv = (const T *)c->gvel;
   const T *wl = (const T *)c->wls;
-  const char *nm = MODE == 0 ? "pass_f" : "pass_f_rhs";
+  const char *nm = MODE == 0 ? "pass_f" : (MODE == 1 ? "pass_f_rhs" : "pass_f_res");
 #define PF(NEQ, VISC, HEAT)                                                                                       \
   LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE>), dim3(gr), dim3(kBlock), 0, c->stream, \
                                          c->p, c->nc, U, U0, Unew, c->nbr, c->mir, c->geo, wl, c->vol, c->volcfl,  \
@@ -636,7 +643,19 @@ int do_pass_f_rhs(fk_ctx *c) {
   return FK_OK;
 }
 
+int do_update(fk_ctx *c, int irk, double dt, const double *dtp);
+
+// RK stage irk from pass F: fused (MODE 0) or residual (MODE 2) + rk_update,
+// as c->fuse_update selects; both give the same bits.
 int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp) {
+  if (!c->fuse_update) {
+    int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 2>(c, 0, 0.0, nullptr, nullptr)
+                                        : launch_pass_f_t<double, 2>(c, 0, 0.0, nullptr, nullptr);
+    if (rc) return rc;
+    rc = do_update(c, irk, dt, dtp);
+    c->rhs_valid = false;
+    return rc;
+  }
   const int nb = next_buffer(c);
   int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 0>(c, irk, dt, dtp, c->U[nb])
                                       : launch_pass_f_t<double, 0>(c, irk, dt, dtp, c->U[nb]);
@@ -668,26 +687,44 @@ int do_dt_device(fk_ctx *c) {
   return FK_OK;
 }
 
+// stats of the counters at slots [base, base + 5) of c->lcount
+int read_counts(fk_ctx *c, int base, fk_relax_stats *st) {
+  unsigned long long h[5];
+  HIPCHK(hipMemcpyAsync(h, c->lcount + base, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  st->n_chem = (int64_t)h[0];
+  st->newton_iters = (int64_t)h[1];
+  st->tsat_iters = (int64_t)h[2];
+  st->n_fallback = (int64_t)h[3];
+  st->n_warm = (int64_t)h[4];
+  st->n_therm = 0;  // set by the caller (cells - n_chem, summed over calls)
+  return FK_OK;
+}
+
 int do_relax(fk_ctx *c, fk_relax_stats *st, int mode = FK_RELAX_PLAIN) {
   if (c->p.model != FK_MODEL_TWO_PHASE) return fail(FK_EINVAL, "relaxation needs the two-phase model");
-  if (mode == FK_RELAX_MLP) return fk_relax_mlp_impl(c, st);
-  if (!st) {  // plain relaxation that also emits training pairs, when enabled
-    bool done = false;
-    int rc = fk_relax_emit_impl(c, &done);
-    if (rc || done) return rc;
+  unsigned long long *cnt = (unsigned long long *)c->lcount;
+  if (mode == FK_RELAX_MLP || mode == FK_RELAX_COLD) {
+    int rc = fk_relax_mlp_impl(c, mode == FK_RELAX_MLP);
+    if (rc) return rc;
+  } else {
+    if (!st) {  // plain relaxation that also emits training pairs, when enabled
+      bool done = false;
+      int rc = fk_relax_emit_impl(c, &done);
+      if (rc || done) return rc;
+    }
+    LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_ull, dim3(1), dim3(64), 0, c->stream, cnt, 5));
+    LAUNCH_TIMED(c, "relax", hipLaunchKernelGGL(relax_plain, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, c->p,
+                                                c->nc, c->U[c->ia], cnt));
   }
-  LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_ull, dim3(1), dim3(64), 0, c->stream, (unsigned long long *)c->lcount, 4));
-  LAUNCH_TIMED(c, "relax", hipLaunchKernelGGL(relax_plain, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, c->p,
-                                              c->nc, c->U[c->ia], (unsigned long long *)c->lcount));
   if (st) {
-    unsigned long long h[4];
-    HIPCHK(hipMemcpyAsync(h, c->lcount, sizeof h, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    st->n_chem = (int64_t)h[0];
-    st->n_therm = c->nc - (int64_t)h[0];
-    st->newton_iters = (int64_t)h[1];
-    st->tsat_iters = (int64_t)h[2];
-    st->n_fallback = 0;
+    int rc = read_counts(c, 0, st);
+    if (rc) return rc;
+    st->n_therm = c->nc - st->n_chem;
+  } else {
+    hipLaunchKernelGGL(accum_counts, dim3(1), dim3(64), 0, c->stream, cnt);
+    HIPCHK(hipGetLastError());
+    ++c->relax_calls;
   }
   return FK_OK;
 }
@@ -698,7 +735,7 @@ int check_params(const fk_params *p) {
   if (p->precision != FK_FP64 && p->precision != FK_MIXED) return fail(FK_EINVAL, "bad precision");
   if (p->rkSteps != 1 && p->rkSteps != 3) return fail(FK_EINVAL, "rkSteps must be 1 or 3");
   if (p->adaptDt == 1 && p->adaptDtItr < 1) return fail(FK_EINVAL, "adaptDtItr must be >= 1");
-  if (p->relaxMode != FK_RELAX_PLAIN && p->relaxMode != FK_RELAX_MLP) return fail(FK_EINVAL, "bad relaxMode");
+  if (p->relaxMode < FK_RELAX_PLAIN || p->relaxMode > FK_RELAX_COLD) return fail(FK_EINVAL, "bad relaxMode");
   return FK_OK;
 }
 
@@ -852,7 +889,8 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
   HIPCHK(hipMalloc(&c->dscal, 16 * 8));
   HIPCHK(hipMalloc(&c->dgrad, cs * neq * 3 * 8));
   HIPCHK(hipMalloc(&c->dtheta, cs * neq * 8));
-  HIPCHK(hipMalloc(&c->lcount, 8 * 8));
+  HIPCHK(hipMalloc(&c->lcount, 16 * 8));
+  HIPCHK(hipMemset(c->lcount, 0, 16 * 8));
   HIPCHK(hipMemcpy(c->nbr, hnbr.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->mir, hmir.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->vol, volume, cs * 8, hipMemcpyHostToDevice));
@@ -947,9 +985,36 @@ int fk_compute_rhs_update(fk_ctx *c, int irk, double dt) {
 int fk_relaxation(fk_ctx *c, int mode, fk_relax_stats *st) {
   int rc = require(c);
   if (rc) return rc;
-  if (mode != FK_RELAX_PLAIN && mode != FK_RELAX_MLP) return fail(FK_EINVAL, "unknown relaxation mode");
+  if (mode != FK_RELAX_PLAIN && mode != FK_RELAX_MLP && mode != FK_RELAX_COLD)
+    return fail(FK_EINVAL, "unknown relaxation mode");
   HIPCHK(hipSetDevice(c->device));
   return do_relax(c, st, mode);
+}
+
+int fk_set_tuning(fk_ctx *c, const char *name, int64_t value) {
+  if (!c || !name) return fail(FK_EINVAL, "null argument");
+  if (std::strcmp(name, "fuse_update") == 0) {
+    c->fuse_update = value != 0;
+    return FK_OK;
+  }
+  return fail(FK_EINVAL, std::string("unknown tuning option ") + name);
+}
+
+int fk_relax_totals(fk_ctx *c, fk_relax_stats *st, int reset) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!st && !reset) return fail(FK_EINVAL, "null stats");
+  HIPCHK(hipSetDevice(c->device));
+  if (st) {
+    if ((rc = read_counts(c, 8, st))) return rc;
+    st->n_therm = c->relax_calls * c->nc - st->n_chem;
+  }
+  if (reset) {
+    hipLaunchKernelGGL(zero_ull, dim3(1), dim3(64), 0, c->stream, (unsigned long long *)c->lcount + 8, 5);
+    HIPCHK(hipGetLastError());
+    c->relax_calls = 0;
+  }
+  return FK_OK;
 }
 
 int fk_cons_calc(fk_ctx *c, double *out) {

@@ -372,81 +372,152 @@ __global__ void __launch_bounds__(256) mlp_label_k(fk_params P, int64_t n, const
   fk::mlp_targets(P, u, y + c * 3);
 }
 
-// Surrogate relaxation: MLP warm start (wave-cooperative inference of 64
-// cells), exact classification by the sign of f(T_L; p_L), FP64 Newton
-// polish, plain reference solve when the polish fails.
-// counts: [n_chem, newton iterations, 0, fallbacks].
-__global__ void __launch_bounds__(256) relax_mlp_k(fk_params P, int64_t nc, double *__restrict__ U,
-                                                   const float *__restrict__ w, const uint16_t *__restrict__ wbg,
-                                                   Norm nm, unsigned long long *__restrict__ counts) {
-  __shared__ __attribute__((aligned(16))) uint16_t wb[8192];
-  __shared__ FwdLds lds[4];
-  load_wb(wbg, wb, 8192);
-  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-  unsigned long long nchem = 0, nit = 0, nfb = 0;
-  const int64_t nchunk = (nc + 63) / 64;
-  const int64_t per = (nchunk + 3) / 4 * 4;
-  for (int64_t ch = (int64_t)blockIdx.x * 4 + wv; ch < per; ch += (int64_t)gridDim.x * 4) {
-    const int64_t c = ch * 64 + l;
-    const bool live = c < nc;
-    double u[10];
-    for (int v = 0; v < 10; ++v) u[v] = live ? U[c * 10 + v] : (v == 0 ? 0.5 : (v == 1 || v == 2 ? 1.0 : 0.0));
-    float xf[3];
-    fk::mlp_features(P, u, xf);
-    float yw[3];
-    for (int t = 0; t < 4; ++t) {
-      if ((l >> 4) == t)
-        for (int j = 0; j < 3; ++j) lds[wv].x[(l & 15) * 3 + j] = (xf[j] - nm.xm[j]) / nm.xs[j];
-      __builtin_amdgcn_s_waitcnt(0);
-      __builtin_amdgcn_wave_barrier();
-      mlp_fwd16(w, wb, lds[wv], nullptr, nullptr, nullptr);
-      if ((l >> 4) == t)
-        for (int j = 0; j < 3; ++j) yw[j] = lds[wv].y[(l & 15) * 3 + j] * nm.ys[j] + nm.ym[j];
-      __builtin_amdgcn_wave_barrier();
-    }
-    if (!live) continue;
-    // warm start (T*, p*, alpha*)
-    const double Ts = (double)yw[0], ps = (double)kPScale * sinh((double)yw[1]);
-    const double as = fmin(fmax((double)yw[2], 1e-12), 1.0 - 1e-12);
-    int chem = 0;
+// Surrogate relaxation, stage 1 (every cell, one thread each): exact
+// classification by the sign of f(T_L; p_L); thermal cells try at most
+// kColdIters analytic-Jacobian Newton iterations from the cell's own state
+// (one hydro step away from the previous equilibrium, so most converge in 2-3)
+// and are written back when converged.  Chemical cells and thermal cells that
+// did not converge go to the work list as (cell << 1) | chem for stage 2.
+// counts: [n_chem, newton iterations, -, fallbacks, list length].
+constexpr int kColdIters = 4;
+
+__global__ void __launch_bounds__(256) relax_screen_k(fk_params P, int64_t nc, double *__restrict__ U,
+                                                      uint32_t *__restrict__ list,
+                                                      unsigned long long *__restrict__ counts) {
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int l = threadIdx.x & 63;
+  int chem = 0, it = 0, push = 0;
+  if (c < nc) {
+    double u[9];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) u[v] = U[c * 10 + v];
     if (P.chemFlag == 1) {
       const double avf = fk::clampd(P, u[0]);
       const double rho = u[1] / avf, enr = u[6] / (avf * rho);
       const double tL = fk::eos_T(P, 0, rho, enr), pL = fk::eos_p(P, 0, rho, enr);
       chem = fk::sat_fun(P, pL, tL) > 0 && avf > P.chemLimit;
     }
+    if (chem) {
+      push = 1;
+    } else {
+      fk::ThermProb t;
+      fk::therm_setup(P, u, t);
+      double xr, xe;
+      fk::therm_cold(P, u, t, xr, xe);
+      it = fk::therm_newton(P, t, xr, xe, P.polishTol, kColdIters);
+      if (it > 0) {
+        fk::therm_commit(P, t, xr, xe, u);
+        U[c * 10 + 0] = u[0];
+        U[c * 10 + 6] = u[6];
+        U[c * 10 + 7] = u[7];
+      } else {
+        it = kColdIters;
+        push = 1;
+      }
+    }
+  }
+  // wave-aggregated append to the list (its order does not affect results:
+  // every cell is solved independently)
+  const unsigned long long mask = __ballot(push);
+  const int npush = __popcll(mask);
+  unsigned long long base = 0;
+  const int leader = npush ? __ffsll((long long)mask) - 1 : 0;
+  if (npush && l == leader) base = atomicAdd(&counts[4], (unsigned long long)npush);
+  base = __shfl(base, leader, 64);
+  if (push) {
+    const int rank = __popcll(mask & ((1ull << l) - 1ull));
+    list[base + rank] = (uint32_t)((c << 1) | chem);
+  }
+  unsigned long long a = chem, b = it;
+  for (int off = 32; off > 0; off >>= 1) {
+    a += __shfl_down(a, off, 64);
+    b += __shfl_down(b, off, 64);
+  }
+  if (l == 0) {
+    if (a) atomicAdd(&counts[0], a);
+    atomicAdd(&counts[1], b);
+  }
+}
+
+// Surrogate relaxation, stage 2 (the work list, 64 entries per wave): MLP
+// warm start by wave-cooperative inference (WARM), or the cell state (the
+// ablation without the network), FP64 Newton polish, and the plain reference
+// solve from the cell state when the polish fails.
+template <bool WARM>
+__global__ void __launch_bounds__(256) relax_list_k(fk_params P, double *__restrict__ U,
+                                                    const uint32_t *__restrict__ list, const float *__restrict__ w,
+                                                    const uint16_t *__restrict__ wbg, Norm nm,
+                                                    unsigned long long *__restrict__ counts) {
+  __shared__ __attribute__((aligned(16))) uint16_t wb[8192];
+  __shared__ FwdLds lds[4];
+  const int64_t n = (int64_t)counts[4];
+  const int64_t nchunk = (n + 63) / 64;
+  if ((int64_t)blockIdx.x * 4 >= nchunk) return;  // whole block idle: skip the weight load
+  if (WARM) load_wb(wbg, wb, 8192);
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  unsigned long long nit = 0, nfb = 0;
+  const int64_t per = (nchunk + 3) / 4 * 4;
+  for (int64_t ch = (int64_t)blockIdx.x * 4 + wv; ch < per; ch += (int64_t)gridDim.x * 4) {
+    const int64_t e = ch * 64 + l;
+    const bool live = e < n;
+    const uint32_t ent = live ? list[e] : 0u;
+    const int64_t c = (int64_t)(ent >> 1);
+    const int chem = (int)(ent & 1u);
+    double u[10];
+    for (int v = 0; v < 10; ++v) u[v] = live ? U[c * 10 + v] : (v == 0 ? 0.5 : (v == 1 || v == 2 ? 1.0 : 0.0));
     const double a0 = fk::clampd(P, u[0]);
     const int pm = (1 - a0 > a0) ? 1 : 0, k = 1 - pm;
-    const double rk = (ps + P.pInf[k]) / ((P.Cp[k] / P.gamma[k]) * Ts * (P.gamma[k] - 1));  // rho_k(p*, T*)
-    const double ek = (ps + P.gamma[k] * P.pInf[k] + (P.gamma[k] - 1) * P.eta[k] * rk) / (rk * (P.gamma[k] - 1));
+    double as, rk, ek;  // start (alpha_k, rho_k, e_k)
+    if constexpr (WARM) {
+      float xf[3];
+      fk::mlp_features(P, u, xf);
+      float yw[3];
+      for (int t = 0; t < 4; ++t) {
+        if ((l >> 4) == t)
+          for (int j = 0; j < 3; ++j) lds[wv].x[(l & 15) * 3 + j] = (xf[j] - nm.xm[j]) / nm.xs[j];
+        __builtin_amdgcn_s_waitcnt(0);
+        __builtin_amdgcn_wave_barrier();
+        mlp_fwd16(w, wb, lds[wv], nullptr, nullptr, nullptr);
+        if ((l >> 4) == t)
+          for (int j = 0; j < 3; ++j) yw[j] = lds[wv].y[(l & 15) * 3 + j] * nm.ys[j] + nm.ym[j];
+        __builtin_amdgcn_wave_barrier();
+      }
+      // (T*, p*, alpha_L*) -> (alpha_k, rho_k(p*, T*), e_k(p*, rho_k))
+      const double Ts = (double)yw[0], ps = (double)kPScale * sinh((double)yw[1]);
+      const double aL = fmin(fmax((double)yw[2], 1e-12), 1.0 - 1e-12);
+      as = k == 0 ? aL : 1 - aL;
+      rk = (ps + P.pInf[k]) / ((P.Cp[k] / P.gamma[k]) * Ts * (P.gamma[k] - 1));
+      ek = (ps + P.gamma[k] * P.pInf[k] + (P.gamma[k] - 1) * P.eta[k] * rk) / (rk * (P.gamma[k] - 1));
+    } else {
+      as = k == 0 ? a0 : 1 - a0;
+      rk = k == 0 ? u[1] / a0 : u[2] / (1 - a0);
+      if (chem) {
+        ek = k == 0 ? u[6] / u[1] : u[7] / u[2];
+      } else {
+        fk::ThermProb t;
+        fk::therm_setup(P, u, t);
+        ek = t.enr[k];
+      }
+    }
+    if (!live) continue;
     double uu[10];
     for (int v = 0; v < 10; ++v) uu[v] = u[v];
-    int it;
-    if (chem) it = fk::chem_polish(P, uu, k == 0 ? as : 1 - as, rk, ek, P.polishTol, 30);
-    else it = fk::therm_polish(P, uu, rk, ek, P.polishTol, 30);
+    int it = chem ? fk::chem_polish(P, uu, as, rk, ek, P.polishTol, 30)
+                  : fk::therm_polish(P, uu, rk, ek, P.polishTol, 30);
     if (it < 0) {  // fallback: the reference solve from the cell state
       ++nfb;
+      nit += 30;
       for (int v = 0; v < 10; ++v) uu[v] = u[v];
-      if (P.chemFlag == 1) {
-        int ts;
-        const double avf = fk::clampd(P, u[0]);
-        const double rho = u[1] / avf, enr = u[6] / (avf * rho);
-        const double tL = fk::eos_T(P, 0, rho, enr), pL = fk::eos_p(P, 0, rho, enr);
-        chem = tL > fk::tsat_dev(P, pL, tL, P.satTempUrf, &ts) && avf > P.chemLimit;
-      }
       it = chem ? fk::chem_dev(P, uu) : fk::therm_dev(P, uu);
     }
-    nchem += chem;
     nit += it;
     for (int v = 0; v < 10; ++v) U[c * 10 + v] = uu[v];
   }
   for (int off = 32; off > 0; off >>= 1) {
-    nchem += __shfl_down(nchem, off, 64);
     nit += __shfl_down(nit, off, 64);
     nfb += __shfl_down(nfb, off, 64);
   }
   if (l == 0) {
-    atomicAdd(&counts[0], nchem);
     atomicAdd(&counts[1], nit);
     atomicAdd(&counts[3], nfb);
   }
@@ -486,6 +557,8 @@ __global__ void __launch_bounds__(256) relax_emit_k(fk_params P, int64_t nc, dou
 __global__ void zero_u64_k(unsigned long long *p, int n) {
   if ((int)threadIdx.x < n) p[threadIdx.x] = 0ull;
 }
+
+
 
 // ---------------- host side ----------------
 
@@ -543,31 +616,35 @@ int ensure_data(fk_ctx *c, int64_t cap) {
 
 }  // namespace
 
-// called from the relaxation entry point of fk_kernels.hip
-int fk_relax_mlp_impl(fk_ctx *c, fk_relax_stats *st) {
+// called from the relaxation entry point of fk_kernels.hip; counts layout as
+// relax_screen_k / relax_list_k (slots [0, 5) of c->lcount, zeroed here).
+int fk_relax_mlp_impl(fk_ctx *c, bool warm) {
   Mlp &m = c->mlp;
-  if (!m.ready) return fail(FK_ESTATE, "surrogate relaxation needs trained or loaded MLP weights");
+  if (warm && !m.ready) return fail(FK_ESTATE, "surrogate relaxation needs trained or loaded MLP weights");
   // the exact classification needs f(T) increasing: B <= 0 and C >= 0
   const fk_params &P = c->p;
   if (P.chemFlag == 1 && !((P.eta[0] - P.eta[1]) <= 0 && (P.Cp[1] - P.Cp[0]) >= 0))
     return fail(FK_EINVAL, "surrogate classification needs eta_L <= eta_G and Cp_G >= Cp_L");
-  LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_u64_k, dim3(1), dim3(64), 0, c->stream,
-                                             (unsigned long long *)c->lcount, 4));
-  const int64_t nchunk = (c->nc + 63) / 64;
-  const unsigned grid = (unsigned)std::min<int64_t>((nchunk + 3) / 4, 8192);
-  LAUNCH_TIMED(c, "relax_mlp", hipLaunchKernelGGL(relax_mlp_k, dim3(grid), dim3(256), 0, c->stream, c->p, c->nc,
-                                                  c->U[c->ia], m.w, m.wb, norm_of(m),
-                                                  (unsigned long long *)c->lcount));
-  if (st) {
-    unsigned long long h[4];
-    HIPCHK(hipMemcpyAsync(h, c->lcount, sizeof h, hipMemcpyDeviceToHost, c->stream));
-    HIPCHK(hipStreamSynchronize(c->stream));
-    st->n_chem = (int64_t)h[0];
-    st->n_therm = c->nc - (int64_t)h[0];
-    st->newton_iters = (int64_t)h[1];
-    st->tsat_iters = 0;
-    st->n_fallback = (int64_t)h[3];
+  if (c->nc >= (int64_t(1) << 31)) return fail(FK_ERANGE, "surrogate relaxation supports < 2^31 cells");
+  if (m.list_cap < c->nc) {
+    if (m.list) (void)hipFree(m.list);
+    m.list = nullptr;
+    m.list_cap = 0;
+    HIPCHK(hipMalloc(&m.list, (size_t)c->nc * sizeof(uint32_t)));
+    m.list_cap = c->nc;
   }
+  unsigned long long *cnt = (unsigned long long *)c->lcount;
+  LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_u64_k, dim3(1), dim3(64), 0, c->stream, cnt, 5));
+  LAUNCH_TIMED(c, "relax_screen", hipLaunchKernelGGL(relax_screen_k, dim3((unsigned)((c->nc + 255) / 256)), dim3(256),
+                                                     0, c->stream, c->p, c->nc, c->U[c->ia], m.list, cnt));
+  const int64_t nchunk = (c->nc + 63) / 64;
+  const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nchunk + 3) / 4, 1024));
+  if (warm)
+    LAUNCH_TIMED(c, "relax_list", hipLaunchKernelGGL(relax_list_k<true>, dim3(grid), dim3(256), 0, c->stream, c->p,
+                                                     c->U[c->ia], m.list, m.w, m.wb, norm_of(m), cnt));
+  else
+    LAUNCH_TIMED(c, "relax_list", hipLaunchKernelGGL(relax_list_k<false>, dim3(grid), dim3(256), 0, c->stream, c->p,
+                                                     c->U[c->ia], m.list, m.w, m.wb, norm_of(m), cnt));
   return FK_OK;
 }
 
@@ -590,7 +667,7 @@ int fk_relax_emit_impl(fk_ctx *c, bool *done) {
 
 void fk_mlp_free(fk_ctx *c) {
   Mlp &m = c->mlp;
-  void *ptrs[] = {m.w, m.m, m.v, m.wb, m.grad, m.part, m.dx, m.dy};
+  void *ptrs[] = {m.w, m.m, m.v, m.wb, m.grad, m.part, m.dx, m.dy, m.list};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   m = Mlp{};

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/fooska.h"
+#include "fk_math.h"
 
 namespace fk {
 
@@ -24,7 +25,7 @@ __device__ inline double eos_T(const fk_params &P, int k, double rho, double e) 
 __device__ inline double tsat_dev(const fk_params &P, double Pr, double temp, double urf, int *iters) {
   const double Pinf = P.pInf[0], g1 = P.gamma[0], g2 = P.gamma[1], Cp1 = P.Cp[0], Cp2 = P.Cp[1];
   const double Cv1 = Cp1 / g1, Cv2 = Cp2 / g2;
-  const double et2 = P.satConst[0] * log10(Pr) + P.satConst[1];
+  const double et2 = P.satConst[0] * fkmath::log10(Pr) + P.satConst[1];
   const double As = (Cp1 - Cp2 + et2 - 0.0) / (Cp2 - Cv2);
   const double Bs = (P.eta[0] - P.eta[1]) / (Cp2 - Cv2);
   const double Cs = (Cp2 - Cp1) / (Cp2 - Cv2);
@@ -33,8 +34,8 @@ __device__ inline double tsat_dev(const fk_params &P, double Pr, double temp, do
   double fun = 1, T = temp;
   int itr = 0;
   while (fabs(fun) > 1.e-2 && itr < 50) {
-    fun = As + Bs / T + Cs * log10(T) + Ds * log10(Pr + Pinf) - log10(Pr);
-    double fd = As + Bs / (T + dt) + Cs * log10(T + dt) + Ds * log10(Pr + Pinf) - log10(Pr);
+    fun = As + Bs / T + Cs * fkmath::log10(T) + Ds * fkmath::log10(Pr + Pinf) - fkmath::log10(Pr);
+    double fd = As + Bs / (T + dt) + Cs * fkmath::log10(T + dt) + Ds * fkmath::log10(Pr + Pinf) - fkmath::log10(Pr);
     double gr = (fd - fun) / dt;
     T = T - urf * fun / gr;
     itr++;
@@ -138,10 +139,12 @@ __device__ inline int therm_dev(const fk_params &P, double *U) {
   return itr;
 }
 
+// Gibbs energy of chemRlx.cu; log10(T^g / (p + pinf)^(g - 1)) taken as
+// g log10(T) - (g - 1) log10(p + pinf).
 __device__ inline double gibbs_dev(const fk_params &P, int k, double pr, double T) {
-  const double et = k == 0 ? 0.0 : P.satConst[0] * log10(pr) + P.satConst[1];
+  const double et = k == 0 ? 0.0 : P.satConst[0] * fkmath::log10(pr) + P.satConst[1];
   const double h = P.Cp[k] * T + P.eta[k];
-  const double s = (P.Cp[k] / P.gamma[k]) * log10(pow(T, P.gamma[k]) / pow(pr + P.pInf[k], P.gamma[k] - 1)) + et;
+  const double s = (P.Cp[k] / P.gamma[k]) * (P.gamma[k] * fkmath::log10(T) - (P.gamma[k] - 1) * fkmath::log10(pr + P.pInf[k])) + et;
   return h - T * s;
 }
 
@@ -301,44 +304,70 @@ __device__ inline int chem_dev(const fk_params &P, double *U) {
 __device__ inline double sat_fun(const fk_params &P, double Pr, double T) {
   const double g1 = P.gamma[0], g2 = P.gamma[1], Cp1 = P.Cp[0], Cp2 = P.Cp[1];
   const double Cv1 = Cp1 / g1, Cv2 = Cp2 / g2;
-  const double et2 = P.satConst[0] * log10(Pr) + P.satConst[1];
+  const double et2 = P.satConst[0] * fkmath::log10(Pr) + P.satConst[1];
   const double As = (Cp1 - Cp2 + et2) / (Cp2 - Cv2);
   const double Bs = (P.eta[0] - P.eta[1]) / (Cp2 - Cv2);
   const double Cs = (Cp2 - Cp1) / (Cp2 - Cv2);
   const double Ds = (Cp1 - Cv1) / (Cp2 - Cv2);
-  return As + Bs / T + Cs * log10(T) + Ds * log10(Pr + P.pInf[0]) - log10(Pr);
+  return As + Bs / T + Cs * fkmath::log10(T) + Ds * fkmath::log10(Pr + P.pInf[0]) - fkmath::log10(Pr);
 }
 
-// Thermal (p-T) equilibrium of the non-primary phase k by Newton with the
-// analytic Jacobian, from the warm start (rho_k, e_k), to |dx_i| <= tol |x_i|.
-// Same unknowns, closure and residuals as thermalRlx (thermRlx.cu:184-204).
-// Returns iterations (> 0) or -1 when it fails (the caller falls back).
-__device__ inline int therm_polish(const fk_params &P, double *U, double rk0, double ek0, double tol, int itmax) {
+// Thermal (p-T) equilibrium problem of one cell as thermalRlx sets it up
+// (thermRlx.cu:4-40, 205-210): partial densities ar_k, phasic energies
+// shifted by the defect de of the mixture internal energy, primary phase pm
+// (larger volume fraction) and the solved-for phase k = 1 - pm.  The unknowns
+// are (rho_k, e_k); the primary phase follows from the volume and energy
+// closure (thermRlx.cu:75-81).
+struct ThermProb {
+  double ar[2], enr[2];
+  int pm, k;
+};
+
+__device__ inline void therm_setup(const fk_params &P, const double *U, ThermProb &t) {
   const double a0 = clampd(P, U[0]), a1 = 1 - a0;
   const double r0 = U[1] / a0, r1 = U[2] / a1;
-  double ar[2] = {a0 * r0, a1 * r1};
-  double enr[2] = {U[6] / ar[0], U[7] / ar[1]};
-  const double rhoMix = ar[0] + ar[1];
+  t.ar[0] = a0 * r0;
+  t.ar[1] = a1 * r1;
+  t.enr[0] = U[6] / t.ar[0];
+  t.enr[1] = U[7] / t.ar[1];
+  const double rhoMix = t.ar[0] + t.ar[1];
   const double v0 = U[3] / rhoMix, v1 = U[4] / rhoMix, v2 = U[5] / rhoMix;
-  const double de = U[8] / rhoMix - (ar[0] / rhoMix) * enr[0] - (ar[1] / rhoMix) * enr[1] - 0.5 * (v0 * v0 + v1 * v1 + v2 * v2);
-  enr[0] += de;
-  enr[1] += de;
-  const int pm = (a1 > a0) ? 1 : 0, k = 1 - pm;
-  double xr = rk0, xe = ek0;
-  if (!(xr > ar[k]) || !isfinite(xe)) return -1;
+  const double de = U[8] / rhoMix - (t.ar[0] / rhoMix) * t.enr[0] - (t.ar[1] / rhoMix) * t.enr[1] -
+                    0.5 * (v0 * v0 + v1 * v1 + v2 * v2);
+  t.enr[0] += de;
+  t.enr[1] += de;
+  t.pm = (a1 > a0) ? 1 : 0;
+  t.k = 1 - t.pm;
+}
+
+// Cold start of the thermal problem: the cell's own (rho_k, e_k + de).
+__device__ inline void therm_cold(const fk_params &P, const double *U, const ThermProb &t, double &xr, double &xe) {
+  const double a0 = clampd(P, U[0]);
+  xr = t.k == 0 ? U[1] / a0 : U[2] / (1 - a0);
+  xe = t.enr[t.k];
+}
+
+// Newton iterations with the analytic Jacobian on (rho_k, e_k) from (xr, xe)
+// until |d rho_k| <= tol |rho_k| and |d e_k| <= tol |e_k|.  Returns the
+// iterations (> 0), or -1 (no convergence within itmax, singular Jacobian,
+// or an iterate outside rho_k > ar_k); xr, xe hold the last iterate.
+__device__ inline int therm_newton(const fk_params &P, const ThermProb &t, double &xr, double &xe, double tol,
+                                   int itmax) {
+  const int pm = t.pm, k = t.k;
+  if (!(xr > t.ar[k]) || !isfinite(xe)) return -1;
   const double sk = k == 0 ? 1.0 : -1.0;  // sign of phase k in r = q_0 - q_1
+  const double gk = P.gamma[k], gm = P.gamma[pm];
+  const double cvk = P.Cp[k] / gk, cvm = P.Cp[pm] / gm;
   for (int it = 1; it <= itmax; ++it) {
-    const double rm = ar[pm] / (1 - ar[k] / xr);
-    const double em = enr[pm] - (ar[k] / ar[pm]) * (xe - enr[k]);
+    const double rm = t.ar[pm] / (1 - t.ar[k] / xr);
+    const double em = t.enr[pm] - (t.ar[k] / t.ar[pm]) * (xe - t.enr[k]);
     if (!(rm > 0)) return -1;
-    const double gk = P.gamma[k], gm = P.gamma[pm];
-    const double cvk = P.Cp[k] / gk, cvm = P.Cp[pm] / gm;
     const double pk = eos_p(P, k, xr, xe), pmm = eos_p(P, pm, rm, em);
     const double tk = eos_T(P, k, xr, xe), tm = eos_T(P, pm, rm, em);
     const double f0 = sk * (pk - pmm), f1 = sk * (tk - tm);
-    // d rho_m / d rho_k, d e_m / d e_k of the closure
-    const double drm = -(rm * rm / ar[pm]) * (ar[k] / (xr * xr));
-    const double dem = -ar[k] / ar[pm];
+    // d rho_m / d rho_k and d e_m / d e_k of the closure
+    const double drm = -(rm * rm / t.ar[pm]) * (t.ar[k] / (xr * xr));
+    const double dem = -t.ar[k] / t.ar[pm];
     const double J00 = sk * ((gk - 1) * (xe - P.eta[k]) - (gm - 1) * (em - P.eta[pm]) * drm);
     const double J01 = sk * ((gk - 1) * xr - (gm - 1) * rm * dem);
     const double J10 = sk * (P.pInf[k] / (cvk * xr * xr) - P.pInf[pm] / (cvm * rm * rm) * drm);
@@ -349,21 +378,36 @@ __device__ inline int therm_polish(const fk_params &P, double *U, double rk0, do
     const double dee = (J00 * f1 - J10 * f0) / det;
     xr -= dr;
     xe -= dee;
-    if (!(xr > ar[k]) || !isfinite(xe)) return -1;
-    if (fabs(dr) <= tol * fabs(xr) && fabs(dee) <= tol * fabs(xe)) {
-      const double s1 = ar[k] / xr, s2 = ar[k] * (xe - enr[k]);
-      double x[4];
-      x[k] = xr;
-      x[2 + k] = xe;
-      x[pm] = (1 / (1 - s1)) * ar[pm];
-      x[2 + pm] = enr[pm] - (1 / ar[pm]) * s2;
-      U[0] = clampd(P, ar[0] / x[0]);
-      U[6] = ar[0] * x[2];
-      U[7] = ar[1] * x[3];
-      return it;
-    }
+    if (!(xr > t.ar[k]) || !isfinite(xe)) return -1;
+    if (fabs(dr) <= tol * fabs(xr) && fabs(dee) <= tol * fabs(xe)) return it;
   }
   return -1;
+}
+
+// Writes alpha_L, alpha_L rho_L e_L, alpha_G rho_G e_G of the solution
+// (updateConsVec, thermRlx.cu:172-187).
+__device__ inline void therm_commit(const fk_params &P, const ThermProb &t, double xr, double xe, double *U) {
+  const int pm = t.pm, k = t.k;
+  const double s1 = t.ar[k] / xr, s2 = t.ar[k] * (xe - t.enr[k]);
+  double x[4];
+  x[k] = xr;
+  x[2 + k] = xe;
+  x[pm] = (1 / (1 - s1)) * t.ar[pm];
+  x[2 + pm] = t.enr[pm] - (1 / t.ar[pm]) * s2;
+  U[0] = clampd(P, t.ar[0] / x[0]);
+  U[6] = t.ar[0] * x[2];
+  U[7] = t.ar[1] * x[3];
+}
+
+// Thermal equilibrium by Newton polish from the warm start (rho_k, e_k).
+// Returns iterations (> 0) or -1 (the caller falls back).
+__device__ inline int therm_polish(const fk_params &P, double *U, double rk0, double ek0, double tol, int itmax) {
+  ThermProb t;
+  therm_setup(P, U, t);
+  double xr = rk0, xe = ek0;
+  const int it = therm_newton(P, t, xr, xe, tol, itmax);
+  if (it > 0) therm_commit(P, t, xr, xe, U);
+  return it;
 }
 
 // Chemical (p-T-g) equilibrium of chemRlx (chemRlx.cu:52-76, 195-215) by
@@ -438,14 +482,16 @@ __device__ inline int chem_polish(const fk_params &P, double *U, double ak0, dou
   return -1;
 }
 
-// Surrogate features of a conserved state: (log rho, e, alpha_L) with e the
-// mixture internal energy per unit mass.
+// Surrogate features of a conserved state: (log rho, e, Y_L) with e the
+// mixture internal energy per unit mass and Y_L the liquid mass fraction.
+// (rho, e, Y_L) fix the thermal equilibrium; (rho, e) fix the chemical one.
 __device__ inline void mlp_features(const fk_params &P, const double *U, float *x) {
+  (void)P;
   const double rho = U[1] + U[2];
   const double ke = 0.5 * (U[3] * U[3] + U[4] * U[4] + U[5] * U[5]) / rho;
   x[0] = (float)log(rho);
   x[1] = (float)((U[8] - ke) / rho);
-  x[2] = (float)clampd(P, U[0]);
+  x[2] = (float)(U[1] / rho);
 }
 
 // Targets of a relaxed state: (T_L, asinh(p_L / 1e4), alpha_L).

@@ -94,6 +94,15 @@ class GpuSolver:
         _lib.check_hip(_lib.hip_lib().fk_relaxation(self._h, mode, ctypes.byref(st)))
         return {k: getattr(st, k) for k, _ in st._fields_}
 
+    def set_tuning(self, name, value):
+        _lib.check_hip(_lib.hip_lib().fk_set_tuning(self._h, name.encode(), int(value)))
+
+    def relax_totals(self, reset=False):
+        """Relaxation counts summed over fk_run_steps since the last reset."""
+        st = _lib.FkRelaxStats()
+        _lib.check_hip(_lib.hip_lib().fk_relax_totals(self._h, ctypes.byref(st), 1 if reset else 0))
+        return {k: getattr(st, k) for k, _ in st._fields_}
+
     def cons_calc(self):
         out = np.zeros(self.neq)
         _lib.check_hip(_lib.hip_lib().fk_cons_calc(self._h, _lib.dptr(out)))

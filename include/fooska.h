@@ -38,6 +38,8 @@ extern "C" {
 
 #define FK_RELAX_PLAIN 0 /* reference Newton solves (relaxation.cu, thermRlx.cu, chemRlx.cu) */
 #define FK_RELAX_MLP 1   /* MLP warm start + FP64 Newton polish, plain Newton as fallback */
+#define FK_RELAX_COLD 2  /* FK_RELAX_MLP with the cell state in place of the MLP
+                            start (ablation: what the network adds) */
 
 /* Solver and material constants; field names follow the reference INPUT
  * struct (strdata.h:49-98).  Two-phase: index 0 = liquid, 1 = gas.
@@ -69,9 +71,11 @@ typedef struct fk_ctx fk_ctx;
 
 typedef struct fk_relax_stats {
   int64_t n_therm, n_chem;      /* cells classified thermal / chemical */
-  int64_t newton_iters;         /* Newton iterations summed over cells (therm + chem) */
+  int64_t newton_iters;         /* Newton iterations summed over cells (therm + chem);
+                                   a failed warm-started polish counts as 30 */
   int64_t tsat_iters;           /* saturation-temperature iterations summed */
   int64_t n_fallback;           /* warm-started cells that fell back to the plain solve */
+  int64_t n_warm;               /* cells solved from the MLP warm start (FK_RELAX_MLP) */
 } fk_relax_stats;
 
 const char *fk_last_error(void);
@@ -116,6 +120,11 @@ int fk_update(fk_ctx *ctx, int irk, double dt);
 int fk_compute_rhs_update(fk_ctx *ctx, int irk, double dt);
 /* relaxation (relaxation.cu:77): per-cell phase relaxation, mode FK_RELAX_*. */
 int fk_relaxation(fk_ctx *ctx, int mode, fk_relax_stats *stats);
+/* Counts summed over every relaxation run without a stats pointer (those of
+ * fk_run_steps) since the context was created or last reset; reset != 0
+ * clears them after reading (stats may then be NULL).  No reference
+ * counterpart (diagnostics of the relaxation cost). */
+int fk_relax_totals(fk_ctx *ctx, fk_relax_stats *stats, int reset);
 /* consCalc (consCalc.cu:36): out[i] = sum_c U_i V_c (i = 0 integrates 1 - alpha). */
 int fk_cons_calc(fk_ctx *ctx, double *out);
 
@@ -136,12 +145,22 @@ int fk_sync(fk_ctx *ctx);
  *  "signal" [nElem]           max signal speed per cell                      */
 int fk_download(fk_ctx *ctx, const char *name, void *host, int64_t bytes);
 int fk_set_debug(fk_ctx *ctx, int on);
+/* Implementation choices that do not change results (no reference
+ * counterpart).  "fuse_update" (default 1): the RK stage update runs inside
+ * pass F; 0: pass F stores the residual and a separate streaming kernel
+ * applies the update. */
+int fk_set_tuning(fk_ctx *ctx, const char *name, int64_t value);
 
 /* ---- surrogate of the relaxation Newton solve (no counterpart in the
- * reference; north-star learned part).  A 3-64-64-64-3 tanh MLP maps the
- * cell features (log rho, e, alpha_L) to the equilibrium (T, asinh(p/1e4),
- * alpha_L); FK_RELAX_MLP uses it as the warm start of an FP64 Newton polish
- * and falls back to the plain solve where the polish fails.  Parameters are
+ * reference).  A 3-64-64-64-3 tanh MLP maps the cell features
+ * (log rho, e, Y_L) -- mixture density, internal energy per unit mass, liquid
+ * mass fraction -- to the equilibrium (T, asinh(p/1e4), alpha_L).
+ * FK_RELAX_MLP classifies every cell exactly (sign of the saturation
+ * function), solves thermal cells by at most 4 analytic-Jacobian Newton
+ * iterations from the cell state, and solves the rest (chemical cells and
+ * thermal cells not converged) by an FP64 Newton polish from the MLP warm
+ * start, falling back to the plain reference solve where the polish fails.
+ * Parameters are
  * FK_MLP_PARAMS floats in the order W1[64][3] b1 W2[64][64] b2 W3[64][64] b3
  * W4[3][64] b4; norm[12] = feature mean[3], std[3], target mean[3], std[3]. */
 #define FK_MLP_PARAMS 8771

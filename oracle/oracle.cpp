@@ -9,6 +9,7 @@
 // (DP) or "singlePhase/src/" (SP) of the reference.
 
 #include "oracle.h"
+#include "../fooskagpu_amd/csrc/fk_math.h"
 
 #include <algorithm>
 #include <cmath>
@@ -393,7 +394,7 @@ double tsat(const fk_params &p, double P, double temp, double urf, int *iters) {
   double Pinf = p.pInf[0], g1 = p.gamma[0], g2 = p.gamma[1], Cp1 = p.Cp[0], Cp2 = p.Cp[1];
   double Cv1 = Cp1 / g1, Cv2 = Cp2 / g2, eta1 = p.eta[0], eta2 = p.eta[1];
   double etaTone1 = 0;
-  double etaTone2 = p.satConst[0] * std::log10(P) + p.satConst[1];
+  double etaTone2 = p.satConst[0] * fkmath::log10(P) + p.satConst[1];
   double As = (Cp1 - Cp2 + etaTone2 - etaTone1) / (Cp2 - Cv2);
   double Bs = (eta1 - eta2) / (Cp2 - Cv2);
   double Cs = (Cp2 - Cp1) / (Cp2 - Cv2);
@@ -401,8 +402,8 @@ double tsat(const fk_params &p, double P, double temp, double urf, int *iters) {
   double dt = 1.e-3, fun = 1, T = temp;
   int itr = 0;
   while (std::fabs(fun) > 1.e-2 && itr < 50) {
-    fun = As + Bs / T + Cs * std::log10(T) + Ds * std::log10(P + Pinf) - std::log10(P);
-    double fun_dt = As + Bs / (T + dt) + Cs * std::log10(T + dt) + Ds * std::log10(P + Pinf) - std::log10(P);
+    fun = As + Bs / T + Cs * fkmath::log10(T) + Ds * fkmath::log10(P + Pinf) - fkmath::log10(P);
+    double fun_dt = As + Bs / (T + dt) + Cs * fkmath::log10(T + dt) + Ds * fkmath::log10(P + Pinf) - fkmath::log10(P);
     double grad_f = (fun_dt - fun) / dt;
     T = T - urf * fun / grad_f;
     itr++;
@@ -543,10 +544,13 @@ int therm_cell(const fk_params &p, double *U) {
   return itr;
 }
 
+// Gibbs free energy g_k(p, T) of chemRlx.cu.  The entropy term
+// log10(T^gamma / (p + pinf)^(gamma - 1)) is evaluated as
+// gamma log10(T) - (gamma - 1) log10(p + pinf) (the same quantity, no pow).
 double gibbs(const fk_params &p, int k, double P, double T) {
-  double etaTone[2] = {0, p.satConst[0] * std::log10(P) + p.satConst[1]};
+  double etaTone[2] = {0, p.satConst[0] * fkmath::log10(P) + p.satConst[1]};
   double enthalpy = p.Cp[k] * T + p.eta[k];
-  double entropy = (p.Cp[k] / p.gamma[k]) * std::log10(std::pow(T, p.gamma[k]) / std::pow(P + p.pInf[k], p.gamma[k] - 1)) +
+  double entropy = (p.Cp[k] / p.gamma[k]) * (p.gamma[k] * fkmath::log10(T) - (p.gamma[k] - 1) * fkmath::log10(P + p.pInf[k])) +
                    etaTone[k];
   return enthalpy - T * entropy;
 }
@@ -1250,6 +1254,8 @@ double orc_wale(const double *g, double vol) {
 }
 
 double orc_tsat(const fk_params *p, double P, double T0, double urf, int *iters) { return tsat(*p, P, T0, urf, iters); }
+double orc_log10(double x) { return fkmath::log10(x); }
+double orc_gibbs(const fk_params *p, int k, double P, double T) { return gibbs(*p, k, P, T); }
 
 int orc_relax_cell(const fk_params *p, double *U, int *iters, int *tsat_iters) {
   int it = 0, ts = 0;

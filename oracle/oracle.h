@@ -74,6 +74,9 @@ void orc_bc_ghost(const fk_params *p, int bc, const double *n, double *W);
 double orc_wale(const double *g, double vol);
 /* Saturation temperature (relaxation.cu:4-38) and iteration count. */
 double orc_tsat(const fk_params *p, double P, double T0, double urf, int *iters);
+/* Shared portable log10 of the relaxation and the Gibbs energy g_k(P, T). */
+double orc_log10(double x);
+double orc_gibbs(const fk_params *p, int k, double P, double T);
 /* Relax one cell's conserved state in place; returns 0 thermal, 1 chemical;
  * iters = Newton iterations. */
 int orc_relax_cell(const fk_params *p, double *U, int *iters, int *tsat_iters);

@@ -55,6 +55,8 @@ def lib(omp=False):
         "orc_tsat": (ctypes.c_double, [P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                        ctypes.POINTER(ctypes.c_int)]),
         "orc_relax_cell": (ctypes.c_int, [P, _dp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
+        "orc_log10": (ctypes.c_double, [ctypes.c_double]),
+        "orc_gibbs": (ctypes.c_double, [P, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     }
     for n, (r, a) in sig.items():
         f = getattr(L, n)
@@ -214,6 +216,15 @@ def tsat(params, P, T0, urf=0.8):
     it = ctypes.c_int()
     T = lib().orc_tsat(ctypes.byref(_P(params)), P, T0, urf, ctypes.byref(it))
     return T, it.value
+
+
+def log10(x):
+    """Portable log10 shared by the oracle and the device relaxation."""
+    return lib().orc_log10(float(x))
+
+
+def gibbs(params, k, P, T):
+    return lib().orc_gibbs(ctypes.byref(_P(params)), k, P, T)
 
 
 def relax_cell(params, U):

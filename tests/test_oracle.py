@@ -382,3 +382,35 @@ def test_tsat_agrees_with_bracketing_root():
         root = brentq(f, 200, 900)
         T, it = po.tsat(TP, P, 300.0)
         assert abs(f(T)) <= 1e-2 and T == pytest.approx(root, abs=0.5) and it < 50
+
+
+def test_portable_log10_within_two_ulp_of_libm():
+    """fkmath::log10 (shared by the oracle and the device relaxation) against
+    the C library: log-uniform samples over the double range, values near 1,
+    exact powers of ten and the special cases."""
+    rng = np.random.default_rng(11)
+    xs = np.concatenate([10.0 ** rng.uniform(-307, 308, 20000), 1 + rng.uniform(-0.3, 0.5, 5000),
+                         1 + rng.uniform(-1e-8, 1e-8, 1000), [5e-324, 2.2250738585072014e-308, 0.5, 2.0, 1e5]])
+    worst = 0.0
+    for x in xs:
+        a, b = po.log10(x), math.log10(x)
+        if b != 0:
+            worst = max(worst, abs(a - b) / math.ulp(b))
+    assert worst <= 2.0, worst
+    assert po.log10(1.0) == 0.0
+    for k in range(-20, 21):
+        assert abs(po.log10(10.0 ** k) - k) <= 2 * math.ulp(k if k else 1.0)
+    assert po.log10(0.0) == -math.inf and po.log10(math.inf) == math.inf
+    assert math.isnan(po.log10(-1.0)) and math.isnan(po.log10(math.nan))
+
+
+def test_gibbs_energy_closed_form():
+    """g_k = Cp T + eta - T [(Cp/gamma) log10(T^gamma / (p + pinf)^(gamma-1)) + eta'_k],
+    eta'_liquid = 0, eta'_vapour = A log10(p) + B (chemRlx.cu)."""
+    p = cases.WATER_VAPOUR
+    for k in (0, 1):
+        for P, T in ((1e5, 300.0), (3e3, 290.0), (2.3e6, 450.0)):
+            et = 0.0 if k == 0 else p["satConst"][0] * math.log10(P) + p["satConst"][1]
+            s = (p["Cp"][k] / p["gamma"][k]) * math.log10(T ** p["gamma"][k] / (P + p["pInf"][k]) ** (p["gamma"][k] - 1))
+            g = p["Cp"][k] * T + p["eta"][k] - T * (s + et)
+            assert po.gibbs(p, k, P, T) == pytest.approx(g, rel=1e-12, abs=1e-6)
