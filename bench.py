@@ -62,7 +62,7 @@ def kernel_bytes(name, B, neq):
     }.get(name)
 
 
-def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None, fuse=1):
+def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None, fuse=0):
     """Time `steps` explicitLoop steps on the device (state resident in HBM).
     train: dict of surrogate-training options (done before the timed region)."""
     from fooskagpu_amd import GpuSolver, surrogate
@@ -132,7 +132,7 @@ def main():
     ap.add_argument("--no-mlp", action="store_true", help="skip the surrogate run")
     ap.add_argument("--order", default="morton", choices=["rcm", "morton", "none"], help="cell order of the mesh")
     ap.add_argument("--train-steps", type=int, default=3000)
-    ap.add_argument("--fuse", type=int, default=1, choices=[0, 1],
+    ap.add_argument("--fuse", type=int, default=0, choices=[0, 1],
                     help="RK update inside pass F (1) or residual + separate update kernel (0); same results")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30,
                     help="size of the plain device copy measured as the HBM reference (0: skip)")

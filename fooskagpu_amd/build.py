@@ -24,6 +24,10 @@ ARCH = os.environ.get("FK_OFFLOAD_ARCH", "gfx950")
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
     "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # FP32 divide and sqrt as reciprocal / hardware sqrt (about 1 ulp) instead
+    # of the correctly rounded expansions: only the mixed path and the MLP
+    # compute in FP32; the FP64 path is unaffected
+    "-fno-hip-fp32-correctly-rounded-divide-sqrt",
     "-Wno-unused-command-line-argument",
 ]
 

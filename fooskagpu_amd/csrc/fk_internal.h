@@ -52,9 +52,6 @@ struct Mlp {
   int64_t n_data = 0, cap_data = 0;
   // emission of solver states during plain relaxations
   int64_t emit_stride = 0, emit_phase = 0;
-  // work list of the surrogate relaxation: (cell << 1) | chem
-  uint32_t *list = nullptr;
-  int64_t list_cap = 0;
 };
 
 }  // namespace fkx
@@ -68,6 +65,9 @@ struct fk_ctx {
   bool have_mesh = false, have_state = false, debug = false, timing = false;
   // mesh
   int32_t *nbr = nullptr, *mir = nullptr;
+  // pass G neighbour staging: slot codes, per-workgroup halo lists, table size
+  int32_t *scode = nullptr, *hoff = nullptr, *hcell = nullptr;
+  int halo_tab = 0;
   void *geo = nullptr, *rvec = nullptr, *wls = nullptr;  // T-typed, sized for double
   double *vol = nullptr, *volcfl = nullptr;
   std::vector<int32_t> slot_face;  // host copy of slot -> face id
@@ -78,14 +78,18 @@ struct fk_ctx {
   double *rhs = nullptr, *dtc = nullptr, *red = nullptr, *dscal = nullptr;
   double *dgrad = nullptr, *dtheta = nullptr;
   // relaxation counters: [0, 5) of the last relaxation (n_chem, Newton
-  // iterations, Tsat iterations, fallbacks, warm-started cells), [8, 13)
-  // summed over the relaxations of fk_run_steps
+  // iterations, Tsat iterations, fallbacks, warm-started cells; slot 5 is the
+  // surrogate work-list length), [8, 13) summed over the relaxations of
+  // fk_run_steps
   long long *lcount = nullptr;
   int64_t relax_calls = 0;  // relaxations summed in lcount[8, 13)
+  // relaxation work list (chemical cells / surrogate list), one entry per cell
+  uint32_t *rlist = nullptr;
+  int64_t rlist_cap = 0;
   double dt = 0, time = 0;
   int64_t step = 0;
   bool rhs_valid = false;
-  bool fuse_update = true;  // RK update inside pass F (else residual + rk_update)
+  bool fuse_update = false;  // RK update inside pass F (else residual + rk_update)
   fkx::Timer tm;
   fkx::Mlp mlp;
   // host geometry kept to rebuild the precision-dependent arrays
@@ -94,6 +98,7 @@ struct fk_ctx {
 
 // defined in fk_mlp.hip
 int fk_relax_mlp_impl(fk_ctx *c, bool warm);
+int ensure_rlist(fk_ctx *c);  // defined in fk_kernels.hip
 int fk_relax_emit_impl(fk_ctx *c, bool *done);
 void fk_mlp_free(fk_ctx *c);
 

@@ -62,9 +62,21 @@ constexpr int ngrad() {
 // [x G/8, (x+1) G/8): neighbouring cells (close in the mesh order) then share
 // an L2.  G is a multiple of 8 (padded by grid_xcd).
 constexpr int kXcd = 8;
+// halo cells per workgroup staged in LDS by pass G (more are converted in place)
+constexpr int kHaloCap = 512;
 __device__ __forceinline__ int64_t xcd_block() {
   const int64_t b = blockIdx.x, per = gridDim.x / kXcd;
   return (b % kXcd) * per + b / kXcd;
+}
+
+// cons2prim of a neighbour outside the LDS table (rare): kept out of line so
+// that the face loop of pass G is not register-allocated for it
+template <int NEQ, typename T>
+__device__ __noinline__ void prim_out_of_line(const fk::Mat<double> md, const double *__restrict__ Uc, T *Wn) {
+  double t[NEQ];
+  fk::cons2prim<NEQ>(md, Uc, t);
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) Wn[v] = (T)t[v];
 }
 
 template <typename T>
@@ -72,105 +84,109 @@ __device__ __forceinline__ V4<T> ld4(const void *p, int64_t i) {
   return reinterpret_cast<const V4<T> *>(p)[i];
 }
 
-// Pass G: first-order states, Green-Gauss gradient, neighbour min/max,
-// limiter and MUSCL face states (computePrimVec.cu:38-80,
-// computeGrad.cu:51-91, computeFaceValues.cu:29-113).
-template <int NEQ, typename T, bool VISC, bool HEAT>
-__global__ void __launch_bounds__(kBlock) pass_g(fk_params P, int64_t nc, const double *__restrict__ U,
-                                                 const int32_t *__restrict__ nbr, const int32_t *__restrict__ mir,
-                                                 const void *__restrict__ geo, const void *__restrict__ rvec,
-                                                 const double *__restrict__ vol, T *__restrict__ face,
-                                                 T *__restrict__ gvel, double *__restrict__ dgrad,
-                                                 double *__restrict__ dtheta) {
-  const int64_t c = xcd_block() * kBlock + threadIdx.x;
-  if (c >= nc) return;
-  const fk::Mat<double> md = fk::make_mat<double>(P);
-  const fk::Mat<T> mt = fk::make_mat<T>(P);
-  double Wd[NEQ];
-  fk::cons2prim<NEQ>(md, U + c * NEQ, Wd);
-  T W[NEQ];
+// Face loop of pass G for the variables [V0, V1): Green-Gauss gradient
+// (owner + / neighbour -, 0.5 (W + W_n) n A / V per face, in slot order),
+// neighbour min/max, limiter and MUSCL face states, velocity gradient.
+template <int NEQ, typename T, bool VISC, bool HEAT, int V0, int V1>
+__device__ __forceinline__ void pass_g_group(const fk_params &P, const fk::Mat<double> &md, const fk::Mat<T> &mt,
+                                             int64_t c, const double *__restrict__ U,
+                                             const int32_t *__restrict__ code, const T *sW, int S,
+                                             const void *__restrict__ geo, const void *__restrict__ rvec, T V,
+                                             const T (&W)[NEQ], T *__restrict__ face, T *__restrict__ gvel,
+                                             double *__restrict__ dgrad, double *__restrict__ dtheta) {
+  constexpr int NV = V1 - V0;
+  T g[NV][3], mn[NV], mx[NV];
 #pragma unroll
-  for (int v = 0; v < NEQ; ++v) W[v] = (T)Wd[v];
-  T g[NEQ][3], mn[NEQ], mx[NEQ];
-#pragma unroll
-  for (int v = 0; v < NEQ; ++v) {
-    g[v][0] = g[v][1] = g[v][2] = 0;
-    mn[v] = (T)1e12;
-    mx[v] = (T)-1e12;
+  for (int i = 0; i < NV; ++i) {
+    g[i][0] = g[i][1] = g[i][2] = 0;
+    mn[i] = (T)1e12;
+    mx[i] = (T)-1e12;
   }
-  const T V = (T)vol[c];
+#pragma unroll 1
   for (int j = 0; j < 4; ++j) {
     const int64_t s = c * 4 + j;
-    const int nb = nbr[s];
-    const bool own = mir[s] & 1;
+    const int cd = code[s];
     const V4<T> ga = ld4<T>(geo, s);
     const T n[3] = {ga.x, ga.y, ga.z};
-    T Wn[NEQ];
-    if (nb >= 0) {
-      double t[NEQ];
-      fk::cons2prim<NEQ>(md, U + (int64_t)nb * NEQ, t);
+    T Wn[NV];
+    bool own;
+    if (cd >= 0) {
+      own = cd & 1;
+      const int e = cd >> 1;
 #pragma unroll
-      for (int v = 0; v < NEQ; ++v) Wn[v] = (T)t[v];
+      for (int i = 0; i < NV; ++i) Wn[i] = sW[(V0 + i) * S + e];
     } else {
+      T full[NEQ];
+      if (cd > -8) {
+        own = true;
 #pragma unroll
-      for (int v = 0; v < NEQ; ++v) Wn[v] = W[v];
-      fk::ghost<NEQ, T>(mt, -nb, n, Wn);
+        for (int v = 0; v < NEQ; ++v) full[v] = W[v];
+        fk::ghost<NEQ, T>(mt, -cd, n, full);
+      } else {
+        const int x = -cd - 8;
+        own = x & 1;
+        prim_out_of_line<NEQ, T>(md, U + (int64_t)(x >> 1) * NEQ, full);
+      }
+#pragma unroll
+      for (int i = 0; i < NV; ++i) Wn[i] = full[V0 + i];
     }
 #pragma unroll
-    for (int v = 0; v < NEQ; ++v) {
-      const T o = Wn[v];
-      if (!(mx[v] >= o)) mx[v] = o;
-      if (!(mn[v] <= o)) mn[v] = o;
-      const T avg = (W[v] + Wn[v]) / 2;
+    for (int i = 0; i < NV; ++i) {
+      const T o = Wn[i];
+      if (!(mx[i] >= o)) mx[i] = o;
+      if (!(mn[i] <= o)) mn[i] = o;
+      const T avg = (W[V0 + i] + Wn[i]) / 2;
       if (own) {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) g[v][d] += avg * n[d] * ga.w / V;
+        for (int d = 0; d < 3; ++d) g[i][d] += avg * n[d] * ga.w / V;
       } else {
 #pragma unroll
-        for (int d = 0; d < 3; ++d) g[v][d] += -avg * n[d] * ga.w / V;
+        for (int d = 0; d < 3; ++d) g[i][d] += -avg * n[d] * ga.w / V;
       }
     }
   }
   // limiter: theta = min over faces of clamp(phi, 0, 1)
-  T th[NEQ];
+  T th[NV];
 #pragma unroll
-  for (int v = 0; v < NEQ; ++v) th[v] = (T)1e9;
-  V4<T> rr[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) rr[j] = ld4<T>(rvec, c * 4 + j);
-#pragma unroll
+  for (int i = 0; i < NV; ++i) th[i] = (T)1e9;
+#pragma unroll 1
   for (int j = 0; j < 4; ++j) {
+    const V4<T> r = ld4<T>(rvec, c * 4 + j);
 #pragma unroll
-    for (int v = 0; v < NEQ; ++v) {
-      const T wu = W[v] + rr[j].x * g[v][0] + rr[j].y * g[v][1] + rr[j].z * g[v][2];
+    for (int i = 0; i < NV; ++i) {
+      const T w0 = W[V0 + i];
+      const T wu = w0 + r.x * g[i][0] + r.y * g[i][1] + r.z * g[i][2];
       T phi;
-      if (wu - W[v] != 0)
-        phi = (wu > W[v]) ? (mx[v] - W[v]) / (2 * (wu - W[v])) : (mn[v] - W[v]) / (2 * (wu - W[v]));
+      if (wu - w0 != 0)
+        phi = (wu > w0) ? (mx[i] - w0) / (2 * (wu - w0)) : (mn[i] - w0) / (2 * (wu - w0));
       else
         phi = 1;
       const T tt = fmax((T)0, fmin(phi, (T)1));
-      if (!(th[v] <= tt)) th[v] = tt;
+      if (!(th[i] <= tt)) th[i] = tt;
     }
   }
   // MUSCL face states
-#pragma unroll
+#pragma unroll 1
   for (int j = 0; j < 4; ++j) {
-    T *out = face + (c * 4 + j) * NEQ;
+    const V4<T> r = ld4<T>(rvec, c * 4 + j);
+    T *out = face + (c * 4 + j) * NEQ + V0;
 #pragma unroll
-    for (int v = 0; v < NEQ; ++v) {
-      const T ext = rr[j].x * g[v][0] + rr[j].y * g[v][1] + rr[j].z * g[v][2];
-      out[v] = W[v] + th[v] * ext;
+    for (int i = 0; i < NV; ++i) {
+      const T ext = r.x * g[i][0] + r.y * g[i][1] + r.z * g[i][2];
+      out[i] = W[V0 + i] + th[i] * ext;
     }
   }
   if constexpr (VISC) {
     constexpr int iu = Model<NEQ>::iu;
     constexpr int NG = ngrad<NEQ, HEAT>();
     T *G = gvel + c * NG;
+    if constexpr (iu >= V0 && iu + 3 <= V1) {
 #pragma unroll
-    for (int i = 0; i < 3; ++i)
+      for (int i = 0; i < 3; ++i)
 #pragma unroll
-      for (int d = 0; d < 3; ++d) G[i * 3 + d] = g[iu + i][d];
-    if constexpr (HEAT) {
+        for (int d = 0; d < 3; ++d) G[i * 3 + d] = g[iu - V0 + i][d];
+    }
+    if constexpr (HEAT && V0 == 0 && V1 >= 5) {
       const T gm = (T)P.gamma[0], pi = (T)P.pInf[0];
       const T R = (T)P.Cp[0] * (gm - 1) / gm;
 #pragma unroll
@@ -179,11 +195,76 @@ __global__ void __launch_bounds__(kBlock) pass_g(fk_params P, int64_t nc, const 
   }
   if (dgrad) {
 #pragma unroll
-    for (int v = 0; v < NEQ; ++v) {
-      dtheta[c * NEQ + v] = (double)th[v];
+    for (int i = 0; i < NV; ++i) {
+      dtheta[c * NEQ + V0 + i] = (double)th[i];
 #pragma unroll
-      for (int d = 0; d < 3; ++d) dgrad[(c * NEQ + v) * 3 + d] = (double)g[v][d];
+      for (int d = 0; d < 3; ++d) dgrad[(c * NEQ + V0 + i) * 3 + d] = (double)g[i][d];
     }
+  }
+}
+
+// Pass G: first-order states, Green-Gauss gradient, neighbour min/max,
+// limiter and MUSCL face states (computePrimVec.cu:38-80,
+// computeGrad.cu:51-91, computeFaceValues.cu:29-113).
+//
+// The workgroup stages the primitive states it needs in LDS first: its own
+// kBlock cells, then the halo cells (neighbours outside the workgroup, listed
+// per workgroup on the host, hcell[hoff[b] .. hoff[b+1])), each converted once
+// by cons2prim.  The face loop then reads neighbour states from LDS through
+// the slot code (slot codes are built in fk_upload_mesh):
+//   code >= 0        (table index << 1) | owner flag
+//   -5 <= code <= -1 boundary face with bc = -code
+//   code <= -8       neighbour not in the table (halo beyond kHaloCap):
+//                    -code - 8 = (cell << 1) | owner flag, converted here
+// LDS layout: SoA sW[v * S + e], S = kBlock + table halo size.
+template <int NEQ, typename T, bool VISC, bool HEAT>
+__global__ void __launch_bounds__(kBlock) pass_g(fk_params P, int64_t nc, const double *__restrict__ U,
+                                                 const int32_t *__restrict__ code, const int32_t *__restrict__ hoff,
+                                                 const int32_t *__restrict__ hcell, int S,
+                                                 const void *__restrict__ geo, const void *__restrict__ rvec,
+                                                 const double *__restrict__ vol, T *__restrict__ face,
+                                                 T *__restrict__ gvel, double *__restrict__ dgrad,
+                                                 double *__restrict__ dtheta) {
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  T *sW = reinterpret_cast<T *>(lds_raw);
+  const int64_t lb = xcd_block();
+  const int64_t c = lb * kBlock + threadIdx.x;
+  const bool live = c < nc;
+  const fk::Mat<double> md = fk::make_mat<double>(P);
+  const fk::Mat<T> mt = fk::make_mat<T>(P);
+  T W[NEQ];
+  if (live) {
+    double Wd[NEQ];
+    fk::cons2prim<NEQ>(md, U + c * NEQ, Wd);
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) {
+      W[v] = (T)Wd[v];
+      sW[v * S + threadIdx.x] = W[v];
+    }
+  }
+  {
+    const int h0 = hoff[lb], h1 = hoff[lb + 1];
+    for (int h = h0 + (int)threadIdx.x; h < h1; h += kBlock) {
+      double t[NEQ];
+      fk::cons2prim<NEQ>(md, U + (int64_t)hcell[h] * NEQ, t);
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) sW[v * S + kBlock + (h - h0)] = (T)t[v];
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  const T V = (T)vol[c];
+  // variables in groups (velocity components in one group), so that the
+  // gradient, min/max and limiter state of a group fits in registers
+  if constexpr (NEQ == 10 && sizeof(T) == 8) {
+    pass_g_group<NEQ, T, VISC, HEAT, 0, 3>(P, md, mt, c, U, code, sW, S, geo, rvec, V, W, face, gvel, dgrad, dtheta);
+    pass_g_group<NEQ, T, VISC, HEAT, 3, 6>(P, md, mt, c, U, code, sW, S, geo, rvec, V, W, face, gvel, dgrad, dtheta);
+    pass_g_group<NEQ, T, VISC, HEAT, 6, 10>(P, md, mt, c, U, code, sW, S, geo, rvec, V, W, face, gvel, dgrad, dtheta);
+  } else if constexpr (NEQ == 10) {
+    pass_g_group<NEQ, T, VISC, HEAT, 0, 6>(P, md, mt, c, U, code, sW, S, geo, rvec, V, W, face, gvel, dgrad, dtheta);
+    pass_g_group<NEQ, T, VISC, HEAT, 6, 10>(P, md, mt, c, U, code, sW, S, geo, rvec, V, W, face, gvel, dgrad, dtheta);
+  } else {
+    pass_g_group<NEQ, T, VISC, HEAT, 0, NEQ>(P, md, mt, c, U, code, sW, S, geo, rvec, V, W, face, gvel, dgrad, dtheta);
   }
 }
 
@@ -194,7 +275,7 @@ __global__ void __launch_bounds__(kBlock) pass_g(fk_params P, int64_t nc, const 
 // (computeRHS.cu:78-146, computeViscousStresses.cu:66-174, update.cu:3-103,
 // computeTimeStep.cu:97-159)
 template <int NEQ, typename T, bool VISC, bool HEAT, int MODE>
-__global__ void __launch_bounds__(kBlock) pass_f(fk_params P, int64_t nc, const double *__restrict__ U,
+__global__ void __launch_bounds__(kBlock, (MODE == 1 ? 2 : 3)) pass_f(fk_params P, int64_t nc, const double *__restrict__ U,
                                                  const double *__restrict__ U0, double *__restrict__ Unew,
                                                  const int32_t *__restrict__ nbr, const int32_t *__restrict__ mir,
                                                  const void *__restrict__ geo, const T *__restrict__ wls,
@@ -226,13 +307,10 @@ __global__ void __launch_bounds__(kBlock) pass_f(fk_params P, int64_t nc, const 
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) rhs[v] = 0.0;
   T sig = (T)-1e9;
-  T Wc1[NEQ];  // first-order state of this cell (viscous boundary ghost)
-  if constexpr (VISC) {
-    double t[NEQ];
-    fk::cons2prim<NEQ>(md, Uc, t);
-#pragma unroll
-    for (int v = 0; v < NEQ; ++v) Wc1[v] = (T)t[v];
-  }
+  // inviscid part: HLLC, non-conservative terms, pressure correction (and the
+  // signal speed); the viscous part follows in a second loop over the faces,
+  // which keeps the register footprint of each part separate
+#pragma unroll 1
   for (int j = 0; j < 4; ++j) {
     const int64_t s = c * 4 + j;
     const int nb = nbr[s];
@@ -288,80 +366,112 @@ __global__ void __launch_bounds__(kBlock) pass_f(fk_params P, int64_t nc, const 
       rhs[8] += (double)(sign * (fr.f[8]) * A);
       rhs[9] += (double)(sign * (fr.f[9]) * A);
     }
-    if constexpr (VISC) {
-      const T *Gc = gvel + c * NG;
-      const T *Gn = nb >= 0 ? gvel + (int64_t)nb * NG : Gc;
-      const T *G0 = own ? Gc : Gn;  // owner's gradient first, as (g_o + g_n)/2
-      const T *G1 = own ? Gn : Gc;
-      T gm[3][3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) gm[i][d] = (G0[i * 3 + d] + G1[i * 3 + d]) / 2;
-      T eddy = 0;
-      if (P.turbulence == 1) eddy = fk::wale<T>(gm, wls[s]);
-      const T ux = gm[0][0], uy = gm[0][1], uz = gm[0][2], vx = gm[1][0], vy = gm[1][1], vz = gm[1][2];
-      const T wx = gm[2][0], wy = gm[2][1], wz = gm[2][2];
-      T st[3][3];
-      st[0][0] = (2 * ux - (T)(2.0 / 3.0) * (ux + vy + wz));
-      st[1][1] = (2 * vy - (T)(2.0 / 3.0) * (ux + vy + wz));
-      st[2][2] = (2 * wz - (T)(2.0 / 3.0) * (ux + vy + wz));
-      st[0][1] = st[1][0] = (uy + vx);
-      st[0][2] = st[2][0] = (uz + wx);
-      st[1][2] = st[2][1] = (vz + wy);
-      // side-1 state: reconstructed neighbour, or the first-order ghost on boundaries
-      T R1[NEQ];
-      if (nb >= 0) {
-#pragma unroll
-        for (int v = 0; v < NEQ; ++v) R1[v] = R[v];
-      } else {
-#pragma unroll
-        for (int v = 0; v < NEQ; ++v) R1[v] = Wc1[v];
-        fk::ghost<NEQ, T>(mt, -nb, n, R1);
-      }
-      T mu;
-      if constexpr (NEQ == 5) {
-        const T rhoF = (L[0] + R1[0]) / 2;
-        mu = mt.mu[0] + rhoF * eddy;
-      } else {
-        const T a1 = L[0], r1 = a1 * L[1] + (1 - a1) * L[2];
-        const T a2 = R1[0], r2 = a2 * R1[1] + (1 - a2) * R1[2];
-        const T rhoF = (r1 + r2) / 2, avfF = (a1 + a2) / 2;
-        mu = avfF * mt.mu[0] + (1 - avfF) * mt.mu[1] + rhoF * eddy;
-      }
-      T fm[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) fm[i] = (mu * st[i][0]) * n[0] + (mu * st[i][1]) * n[1] + (mu * st[i][2]) * n[2];
-      T vf[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) vf[i] = (L[iu + i] + R1[iu + i]) / 2;
-      T fe[3];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) {
-        T stv = 0;
-#pragma unroll
-        for (int d = 0; d < 3; ++d) stv += st[i][d] * vf[d];
-        fe[i] = (mu * stv) * n[i];
-      }
-      const T en = fe[0] + fe[1] + fe[2];
-#pragma unroll
-      for (int i = 0; i < 3; ++i) rhs[iu + i] += (double)(own ? fm[i] * A : -(fm[i]) * A);
-      constexpr int ie = NEQ == 5 ? 4 : 8;
-      rhs[ie] += (double)(own ? en * A : -(en) * A);
-      if constexpr (HEAT) {
-        const T k = mt.cp[0] * mu / mt.Pr0;
-        T gf[3];
-#pragma unroll
-        for (int d = 0; d < 3; ++d) gf[d] = (G0[9 + d] + G1[9 + d]) / 2;
-        const T q = k * (gf[0] * n[0] + gf[1] * n[1] + gf[2] * n[2]);
-        rhs[4] += (double)(own ? q * A : -(q) * A);
-      }
-    }
     if constexpr (MODE == 1) {
       const T cl = fk::signal<NEQ, T>(mt, L, n);
       const T cr = nb >= 0 ? fk::signal<NEQ, T>(mt, R, n) : (T)0.0;
       const T cm = fmax(cl, cr);
       if (!(sig >= cm)) sig = cm;
+    }
+  }
+  if constexpr (VISC) {
+    T Wc1[NEQ];  // first-order state of this cell (viscous boundary ghost)
+    {
+      double t[NEQ];
+      fk::cons2prim<NEQ>(md, Uc, t);
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) Wc1[v] = (T)t[v];
+    }
+#pragma unroll 1
+    for (int j = 0; j < 4; ++j) {
+      const int64_t s = c * 4 + j;
+      const int nb = nbr[s];
+      const int mv = mir[s];
+      const bool own = mv & 1;
+      const V4<T> ga = ld4<T>(geo, s);
+      const T n[3] = {ga.x, ga.y, ga.z};
+      const T A = ga.w;
+      T L[NEQ], R[NEQ];
+      const T *Fs = face + s * NEQ;
+      if (nb >= 0) {
+        const T *Fn = face + (int64_t)(mv >> 1) * NEQ;
+#pragma unroll
+        for (int v = 0; v < NEQ; ++v) {
+          L[v] = own ? Fs[v] : Fn[v];
+          R[v] = own ? Fn[v] : Fs[v];
+        }
+      } else {
+#pragma unroll
+        for (int v = 0; v < NEQ; ++v) L[v] = Fs[v];
+      }
+      {
+        const T *Gc = gvel + c * NG;
+        const T *Gn = nb >= 0 ? gvel + (int64_t)nb * NG : Gc;
+        const T *G0 = own ? Gc : Gn;  // owner's gradient first, as (g_o + g_n)/2
+        const T *G1 = own ? Gn : Gc;
+        T gm[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+          for (int d = 0; d < 3; ++d) gm[i][d] = (G0[i * 3 + d] + G1[i * 3 + d]) / 2;
+        T eddy = 0;
+        if (P.turbulence == 1) eddy = fk::wale<T>(gm, wls[s]);
+        const T ux = gm[0][0], uy = gm[0][1], uz = gm[0][2], vx = gm[1][0], vy = gm[1][1], vz = gm[1][2];
+        const T wx = gm[2][0], wy = gm[2][1], wz = gm[2][2];
+        T st[3][3];
+        st[0][0] = (2 * ux - (T)(2.0 / 3.0) * (ux + vy + wz));
+        st[1][1] = (2 * vy - (T)(2.0 / 3.0) * (ux + vy + wz));
+        st[2][2] = (2 * wz - (T)(2.0 / 3.0) * (ux + vy + wz));
+        st[0][1] = st[1][0] = (uy + vx);
+        st[0][2] = st[2][0] = (uz + wx);
+        st[1][2] = st[2][1] = (vz + wy);
+        // side-1 state: reconstructed neighbour, or the first-order ghost on boundaries
+        T R1[NEQ];
+        if (nb >= 0) {
+#pragma unroll
+          for (int v = 0; v < NEQ; ++v) R1[v] = R[v];
+        } else {
+#pragma unroll
+          for (int v = 0; v < NEQ; ++v) R1[v] = Wc1[v];
+          fk::ghost<NEQ, T>(mt, -nb, n, R1);
+        }
+        T mu;
+        if constexpr (NEQ == 5) {
+          const T rhoF = (L[0] + R1[0]) / 2;
+          mu = mt.mu[0] + rhoF * eddy;
+        } else {
+          const T a1 = L[0], r1 = a1 * L[1] + (1 - a1) * L[2];
+          const T a2 = R1[0], r2 = a2 * R1[1] + (1 - a2) * R1[2];
+          const T rhoF = (r1 + r2) / 2, avfF = (a1 + a2) / 2;
+          mu = avfF * mt.mu[0] + (1 - avfF) * mt.mu[1] + rhoF * eddy;
+        }
+        T fm[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) fm[i] = (mu * st[i][0]) * n[0] + (mu * st[i][1]) * n[1] + (mu * st[i][2]) * n[2];
+        T vf[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) vf[i] = (L[iu + i] + R1[iu + i]) / 2;
+        T fe[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+          T stv = 0;
+#pragma unroll
+          for (int d = 0; d < 3; ++d) stv += st[i][d] * vf[d];
+          fe[i] = (mu * stv) * n[i];
+        }
+        const T en = fe[0] + fe[1] + fe[2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) rhs[iu + i] += (double)(own ? fm[i] * A : -(fm[i]) * A);
+        constexpr int ie = NEQ == 5 ? 4 : 8;
+        rhs[ie] += (double)(own ? en * A : -(en) * A);
+        if constexpr (HEAT) {
+          const T k = mt.cp[0] * mu / mt.Pr0;
+          T gf[3];
+#pragma unroll
+          for (int d = 0; d < 3; ++d) gf[d] = (G0[9 + d] + G1[9 + d]) / 2;
+          const T q = k * (gf[0] * n[0] + gf[1] * n[1] + gf[2] * n[2]);
+          rhs[4] += (double)(own ? q * A : -(q) * A);
+        }
+      }
     }
   }
   if constexpr (MODE != 0) {
@@ -443,9 +553,15 @@ __global__ void advance_time(double *__restrict__ scal) {
   if (threadIdx.x == 0) scal[1] += scal[0];
 }
 
-// partitionRelaxation + thermalRlx/chemRlx in one pass: each thread
-// classifies its cell and runs the matching Newton solve in place.
-// counts: [n_chem, newton iterations, tsat iterations] (64-bit).
+// relaxation (relaxation.cu:77-114), reference Newton solves, in two passes:
+//   relax_therm: partitionRelaxation (Tsat Newton, relaxation.cu:45-75) of
+//                every cell; thermal cells run thermalRlx (thermRlx.cu:190)
+//                in place, chemical cells go to a work list
+//   relax_chem:  chemRlx (chemRlx.cu:217) on the work list
+// Each cell is solved independently with the reference arithmetic, so the
+// result does not depend on the list order; the split keeps the register
+// footprint of the chemical solve out of the thermal pass.
+// counts: [n_chem, newton iterations, tsat iterations, -, -, list length].
 using fk::clampd;
 using fk::eos_p;
 using fk::eos_T;
@@ -453,13 +569,16 @@ using fk::tsat_dev;
 using fk::therm_dev;
 using fk::chem_dev;
 
-__global__ void __launch_bounds__(kBlock) relax_plain(fk_params P, int64_t nc, double *__restrict__ U,
+__global__ void __launch_bounds__(kBlock) relax_therm(fk_params P, int64_t nc, double *__restrict__ U,
+                                                      uint32_t *__restrict__ list,
                                                       unsigned long long *__restrict__ counts) {
   const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  const int l = threadIdx.x & 63;
   int chem = 0, it = 0, ts = 0;
   if (c < nc) {
-    double u[10];
-    for (int v = 0; v < 10; ++v) u[v] = U[c * 10 + v];
+    double u[9];
+#pragma unroll
+    for (int v = 0; v < 9; ++v) u[v] = U[c * 10 + v];
     if (P.chemFlag == 1) {
       const double avf = clampd(P, u[0]);
       const double rho = u[1] / avf;
@@ -469,21 +588,54 @@ __global__ void __launch_bounds__(kBlock) relax_plain(fk_params P, int64_t nc, d
       const double Ts = tsat_dev(P, pL, tL, P.satTempUrf, &ts);
       chem = (tL > Ts && avf > P.chemLimit) ? 1 : 0;
     }
-    it = chem ? chem_dev(P, u) : therm_dev(P, u);
-    for (int v = 0; v < 10; ++v) U[c * 10 + v] = u[v];
+    if (!chem) {
+      it = therm_dev(P, u);
+      U[c * 10 + 0] = u[0];
+      U[c * 10 + 6] = u[6];
+      U[c * 10 + 7] = u[7];
+    }
   }
-  // wave-level sums, one atomic per wave
-  unsigned long long a = chem, b = it, d = ts;
+  const unsigned long long mask = __ballot(chem);
+  const int npush = __popcll(mask);
+  if (npush) {
+    const int leader = __ffsll((long long)mask) - 1;
+    unsigned long long base = 0;
+    if (l == leader) base = atomicAdd(&counts[5], (unsigned long long)npush);
+    base = __shfl(base, leader, 64);
+    if (chem) list[base + __popcll(mask & ((1ull << l) - 1ull))] = (uint32_t)c;
+  }
+  unsigned long long b = it, d = ts;
   for (int off = 32; off > 0; off >>= 1) {
-    a += __shfl_down(a, off, 64);
     b += __shfl_down(b, off, 64);
     d += __shfl_down(d, off, 64);
   }
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&counts[0], a);
+  if (l == 0) {
+    if (npush) atomicAdd(&counts[0], (unsigned long long)npush);
     atomicAdd(&counts[1], b);
     atomicAdd(&counts[2], d);
   }
+}
+
+__global__ void __launch_bounds__(kBlock) relax_chem(fk_params P, double *__restrict__ U,
+                                                     const uint32_t *__restrict__ list,
+                                                     unsigned long long *__restrict__ counts) {
+  const int64_t n = (int64_t)counts[5];
+  unsigned long long nit = 0;
+  for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBlock) {
+    const int64_t c = list[e];
+    double u[10];
+#pragma unroll
+    for (int v = 0; v < 10; ++v) u[v] = U[c * 10 + v];
+    nit += chem_dev(P, u);
+    U[c * 10 + 0] = u[0];
+    U[c * 10 + 1] = u[1];
+    U[c * 10 + 2] = u[2];
+    U[c * 10 + 6] = u[6];
+    U[c * 10 + 7] = u[7];
+    U[c * 10 + 9] = u[9];
+  }
+  for (int off = 32; off > 0; off >>= 1) nit += __shfl_down(nit, off, 64);
+  if ((threadIdx.x & 63) == 0 && nit) atomicAdd(&counts[1], nit);
 }
 
 // sum_c U_i V_c per block (consCalc.cu:3-34); i = 0 integrates 1 - alpha.
@@ -568,10 +720,15 @@ int upload_geometry(fk_ctx *c) {
 
 void free_all(fk_ctx *c) {
   void *ptrs[] = {c->nbr, c->mir, c->geo, c->rvec, c->wls, c->vol, c->volcfl, c->U[0], c->U[1], c->U[2], c->face,
-                  c->gvel, c->rhs, c->dtc, c->red, c->dscal, c->dgrad, c->dtheta, c->lcount};
+                  c->gvel, c->rhs, c->dtc, c->red, c->dscal, c->dgrad, c->dtheta, c->lcount, c->scode, c->hoff,
+                  c->hcell, c->rlist};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->nbr = c->mir = nullptr;
+  c->scode = c->hoff = c->hcell = nullptr;
+  c->halo_tab = 0;
+  c->rlist = nullptr;
+  c->rlist_cap = 0;
   c->geo = c->rvec = c->wls = nullptr;
   c->vol = c->volcfl = nullptr;
   c->U[0] = c->U[1] = c->U[2] = nullptr;
@@ -590,9 +747,12 @@ int launch_pass_g_t(fk_ctx *c) {
   const double *U = c->U[c->ia];
   T *face = (T *)c->face;
   T *gv = (T *)c->gvel;
-#define PG(NEQ, VISC, HEAT)                                                                                            \
-  LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g<NEQ, T, VISC, HEAT>), dim3(gr), dim3(kBlock), 0, c->stream, c->p, \
-                                               c->nc, U, c->nbr, c->mir, c->geo, c->rvec, c->vol, face, gv, dg, dth))
+  const int S = kBlock + c->halo_tab;
+  const size_t lds = (size_t)S * c->neq * sizeof(T);
+#define PG(NEQ, VISC, HEAT)                                                                                          \
+  LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g<NEQ, T, VISC, HEAT>), dim3(gr), dim3(kBlock), lds, c->stream,  \
+                                               c->p, c->nc, U, c->scode, c->hoff, c->hcell, S, c->geo, c->rvec,      \
+                                               c->vol, face, gv, dg, dth))
   if (c->neq == 5) {
     if (heat) PG(5, true, true);
     else if (visc) PG(5, true, false);
@@ -701,6 +861,22 @@ int read_counts(fk_ctx *c, int base, fk_relax_stats *st) {
   return FK_OK;
 }
 
+}  // namespace
+
+// work list of the relaxation passes (one uint32 per cell), allocated on first use
+int ensure_rlist(fk_ctx *c) {
+  if (c->nc >= (int64_t(1) << 31)) return fail(FK_ERANGE, "relaxation supports < 2^31 cells");
+  if (c->rlist_cap >= c->nc) return FK_OK;
+  if (c->rlist) (void)hipFree(c->rlist);
+  c->rlist = nullptr;
+  c->rlist_cap = 0;
+  HIPCHK(hipMalloc(&c->rlist, (size_t)c->nc * sizeof(uint32_t)));
+  c->rlist_cap = c->nc;
+  return FK_OK;
+}
+
+namespace {
+
 int do_relax(fk_ctx *c, fk_relax_stats *st, int mode = FK_RELAX_PLAIN) {
   if (c->p.model != FK_MODEL_TWO_PHASE) return fail(FK_EINVAL, "relaxation needs the two-phase model");
   unsigned long long *cnt = (unsigned long long *)c->lcount;
@@ -713,9 +889,14 @@ int do_relax(fk_ctx *c, fk_relax_stats *st, int mode = FK_RELAX_PLAIN) {
       int rc = fk_relax_emit_impl(c, &done);
       if (rc || done) return rc;
     }
-    LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_ull, dim3(1), dim3(64), 0, c->stream, cnt, 5));
-    LAUNCH_TIMED(c, "relax", hipLaunchKernelGGL(relax_plain, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, c->p,
-                                                c->nc, c->U[c->ia], cnt));
+    int rc = ensure_rlist(c);
+    if (rc) return rc;
+    LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_ull, dim3(1), dim3(64), 0, c->stream, cnt, 6));
+    LAUNCH_TIMED(c, "relax", hipLaunchKernelGGL(relax_therm, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, c->p,
+                                                c->nc, c->U[c->ia], c->rlist, cnt));
+    const unsigned gch = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_for(c->nc), 1024));
+    LAUNCH_TIMED(c, "relax_chem", hipLaunchKernelGGL(relax_chem, dim3(gch), dim3(kBlock), 0, c->stream, c->p,
+                                                     c->U[c->ia], c->rlist, cnt));
   }
   if (st) {
     int rc = read_counts(c, 0, st);
@@ -865,6 +1046,40 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
     const int32_t bb = fc2el[2 * f + 1] >= 0 ? fc2el[2 * f + 1] : a;
     c->h_ls[(size_t)s] = 0.325 * std::pow((volume[a] + volume[bb]) / 2, 0.3333333333333);
   }
+  // pass G halo tables: per workgroup of kBlock cells (logical block index,
+  // padded to the XCD-remapped grid), the neighbour cells outside it
+  const int64_t nblk = (int64_t)grid_xcd(nElem);
+  std::vector<int32_t> hoff((size_t)nblk + 1, 0), hcell, scode((size_t)ns);
+  int halo_tab = 0;
+  {
+    std::vector<int64_t> seen((size_t)nElem, -1);
+    std::vector<int32_t> pos((size_t)nElem, 0);
+    for (int64_t b = 0; b < nblk; ++b) {
+      const int64_t c0 = b * kBlock, c1 = std::min<int64_t>(c0 + kBlock, nElem);
+      int h = 0;
+      for (int64_t cc = c0; cc < c1; ++cc)
+        for (int j = 0; j < 4; ++j) {
+          const int64_t sl = cc * 4 + j;
+          const int32_t nb = hnbr[(size_t)sl];
+          const int own = hmir[(size_t)sl] & 1;
+          if (nb < 0) {
+            scode[(size_t)sl] = nb;
+          } else if (nb >= c0 && nb < c1) {
+            scode[(size_t)sl] = (int32_t)(((nb - c0) << 1) | own);
+          } else {
+            if (seen[(size_t)nb] != b) {
+              seen[(size_t)nb] = b;
+              pos[(size_t)nb] = h++;
+              if (h <= kHaloCap) hcell.push_back(nb);
+            }
+            const int p = pos[(size_t)nb];
+            scode[(size_t)sl] = p < kHaloCap ? (int32_t)(((kBlock + p) << 1) | own) : -(8 + ((nb << 1) | own));
+          }
+        }
+      hoff[(size_t)b + 1] = (int32_t)hcell.size();
+      halo_tab = std::max(halo_tab, std::min(h, kHaloCap));
+    }
+  }
   std::vector<double> vcfl((size_t)nElem);
   for (int64_t cc = 0; cc < nElem; ++cc) vcfl[(size_t)cc] = std::pow(volume[cc], 0.3333);
   free_all(c);
@@ -891,6 +1106,13 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
   HIPCHK(hipMalloc(&c->dtheta, cs * neq * 8));
   HIPCHK(hipMalloc(&c->lcount, 16 * 8));
   HIPCHK(hipMemset(c->lcount, 0, 16 * 8));
+  HIPCHK(hipMalloc(&c->scode, (size_t)ns * 4));
+  HIPCHK(hipMalloc(&c->hoff, hoff.size() * 4));
+  HIPCHK(hipMalloc(&c->hcell, std::max<size_t>(1, hcell.size()) * 4));
+  HIPCHK(hipMemcpy(c->scode, scode.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->hoff, hoff.data(), hoff.size() * 4, hipMemcpyHostToDevice));
+  if (!hcell.empty()) HIPCHK(hipMemcpy(c->hcell, hcell.data(), hcell.size() * 4, hipMemcpyHostToDevice));
+  c->halo_tab = halo_tab;
   HIPCHK(hipMemcpy(c->nbr, hnbr.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->mir, hmir.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->vol, volume, cs * 8, hipMemcpyHostToDevice));

@@ -370,6 +370,8 @@ __global__ void __launch_bounds__(256) mlp_label_k(fk_params P, int64_t n, const
   if (chem) fk::chem_dev(P, u);
   else fk::therm_dev(P, u);
   fk::mlp_targets(P, u, y + c * 3);
+  if (!chem)  // the network serves chemical cells only
+    for (int j = 0; j < 3; ++j) y[c * 3 + j] = NAN;
 }
 
 // Surrogate relaxation, stage 1 (every cell, one thread each): exact
@@ -378,7 +380,7 @@ __global__ void __launch_bounds__(256) mlp_label_k(fk_params P, int64_t n, const
 // (one hydro step away from the previous equilibrium, so most converge in 2-3)
 // and are written back when converged.  Chemical cells and thermal cells that
 // did not converge go to the work list as (cell << 1) | chem for stage 2.
-// counts: [n_chem, newton iterations, -, fallbacks, list length].
+// counts: [n_chem, newton iterations, -, fallbacks, warm-started, list length].
 constexpr int kColdIters = 4;
 
 __global__ void __launch_bounds__(256) relax_screen_k(fk_params P, int64_t nc, double *__restrict__ U,
@@ -395,7 +397,7 @@ __global__ void __launch_bounds__(256) relax_screen_k(fk_params P, int64_t nc, d
       const double avf = fk::clampd(P, u[0]);
       const double rho = u[1] / avf, enr = u[6] / (avf * rho);
       const double tL = fk::eos_T(P, 0, rho, enr), pL = fk::eos_p(P, 0, rho, enr);
-      chem = fk::sat_fun(P, pL, tL) > 0 && avf > P.chemLimit;
+      chem = avf > P.chemLimit && fk::sat_superheated(P, pL, tL);
     }
     if (chem) {
       push = 1;
@@ -422,7 +424,7 @@ __global__ void __launch_bounds__(256) relax_screen_k(fk_params P, int64_t nc, d
   const int npush = __popcll(mask);
   unsigned long long base = 0;
   const int leader = npush ? __ffsll((long long)mask) - 1 : 0;
-  if (npush && l == leader) base = atomicAdd(&counts[4], (unsigned long long)npush);
+  if (npush && l == leader) base = atomicAdd(&counts[5], (unsigned long long)npush);
   base = __shfl(base, leader, 64);
   if (push) {
     const int rank = __popcll(mask & ((1ull << l) - 1ull));
@@ -450,12 +452,12 @@ __global__ void __launch_bounds__(256) relax_list_k(fk_params P, double *__restr
                                                     unsigned long long *__restrict__ counts) {
   __shared__ __attribute__((aligned(16))) uint16_t wb[8192];
   __shared__ FwdLds lds[4];
-  const int64_t n = (int64_t)counts[4];
+  const int64_t n = (int64_t)counts[5];
   const int64_t nchunk = (n + 63) / 64;
   if ((int64_t)blockIdx.x * 4 >= nchunk) return;  // whole block idle: skip the weight load
   if (WARM) load_wb(wbg, wb, 8192);
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-  unsigned long long nit = 0, nfb = 0;
+  unsigned long long nit = 0, nfb = 0, nwarm = 0;
   const int64_t per = (nchunk + 3) / 4 * 4;
   for (int64_t ch = (int64_t)blockIdx.x * 4 + wv; ch < per; ch += (int64_t)gridDim.x * 4) {
     const int64_t e = ch * 64 + l;
@@ -468,7 +470,7 @@ __global__ void __launch_bounds__(256) relax_list_k(fk_params P, double *__restr
     const double a0 = fk::clampd(P, u[0]);
     const int pm = (1 - a0 > a0) ? 1 : 0, k = 1 - pm;
     double as, rk, ek;  // start (alpha_k, rho_k, e_k)
-    if constexpr (WARM) {
+    if (WARM) {  // wave-uniform: every lane joins the cooperative inference
       float xf[3];
       fk::mlp_features(P, u, xf);
       float yw[3];
@@ -488,7 +490,8 @@ __global__ void __launch_bounds__(256) relax_list_k(fk_params P, double *__restr
       as = k == 0 ? aL : 1 - aL;
       rk = (ps + P.pInf[k]) / ((P.Cp[k] / P.gamma[k]) * Ts * (P.gamma[k] - 1));
       ek = (ps + P.gamma[k] * P.pInf[k] + (P.gamma[k] - 1) * P.eta[k] * rk) / (rk * (P.gamma[k] - 1));
-    } else {
+    }
+    if (!WARM || !chem) {  // thermal cells (and the ablation): the cell state
       as = k == 0 ? a0 : 1 - a0;
       rk = k == 0 ? u[1] / a0 : u[2] / (1 - a0);
       if (chem) {
@@ -500,6 +503,7 @@ __global__ void __launch_bounds__(256) relax_list_k(fk_params P, double *__restr
       }
     }
     if (!live) continue;
+    nwarm += (WARM && chem) ? 1 : 0;
     double uu[10];
     for (int v = 0; v < 10; ++v) uu[v] = u[v];
     int it = chem ? fk::chem_polish(P, uu, as, rk, ek, P.polishTol, 30)
@@ -516,10 +520,12 @@ __global__ void __launch_bounds__(256) relax_list_k(fk_params P, double *__restr
   for (int off = 32; off > 0; off >>= 1) {
     nit += __shfl_down(nit, off, 64);
     nfb += __shfl_down(nfb, off, 64);
+    nwarm += __shfl_down(nwarm, off, 64);
   }
   if (l == 0) {
     atomicAdd(&counts[1], nit);
     atomicAdd(&counts[3], nfb);
+    atomicAdd(&counts[4], nwarm);
   }
 }
 
@@ -550,6 +556,8 @@ __global__ void __launch_bounds__(256) relax_emit_k(fk_params P, int64_t nc, dou
     if (slot < cap) {
       for (int j = 0; j < 3; ++j) dx[slot * 3 + j] = xf[j];
       fk::mlp_targets(P, u, dy + slot * 3);
+      if (!chem)  // the network serves chemical cells only
+        for (int j = 0; j < 3; ++j) dy[slot * 3 + j] = NAN;
     }
   }
 }
@@ -617,7 +625,7 @@ int ensure_data(fk_ctx *c, int64_t cap) {
 }  // namespace
 
 // called from the relaxation entry point of fk_kernels.hip; counts layout as
-// relax_screen_k / relax_list_k (slots [0, 5) of c->lcount, zeroed here).
+// relax_screen_k / relax_list_k (slots [0, 6) of c->lcount, zeroed here).
 int fk_relax_mlp_impl(fk_ctx *c, bool warm) {
   Mlp &m = c->mlp;
   if (warm && !m.ready) return fail(FK_ESTATE, "surrogate relaxation needs trained or loaded MLP weights");
@@ -625,26 +633,20 @@ int fk_relax_mlp_impl(fk_ctx *c, bool warm) {
   const fk_params &P = c->p;
   if (P.chemFlag == 1 && !((P.eta[0] - P.eta[1]) <= 0 && (P.Cp[1] - P.Cp[0]) >= 0))
     return fail(FK_EINVAL, "surrogate classification needs eta_L <= eta_G and Cp_G >= Cp_L");
-  if (c->nc >= (int64_t(1) << 31)) return fail(FK_ERANGE, "surrogate relaxation supports < 2^31 cells");
-  if (m.list_cap < c->nc) {
-    if (m.list) (void)hipFree(m.list);
-    m.list = nullptr;
-    m.list_cap = 0;
-    HIPCHK(hipMalloc(&m.list, (size_t)c->nc * sizeof(uint32_t)));
-    m.list_cap = c->nc;
-  }
+  int rc = ensure_rlist(c);
+  if (rc) return rc;
   unsigned long long *cnt = (unsigned long long *)c->lcount;
-  LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_u64_k, dim3(1), dim3(64), 0, c->stream, cnt, 5));
+  LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_u64_k, dim3(1), dim3(64), 0, c->stream, cnt, 6));
   LAUNCH_TIMED(c, "relax_screen", hipLaunchKernelGGL(relax_screen_k, dim3((unsigned)((c->nc + 255) / 256)), dim3(256),
-                                                     0, c->stream, c->p, c->nc, c->U[c->ia], m.list, cnt));
+                                                     0, c->stream, c->p, c->nc, c->U[c->ia], c->rlist, cnt));
   const int64_t nchunk = (c->nc + 63) / 64;
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nchunk + 3) / 4, 1024));
   if (warm)
     LAUNCH_TIMED(c, "relax_list", hipLaunchKernelGGL(relax_list_k<true>, dim3(grid), dim3(256), 0, c->stream, c->p,
-                                                     c->U[c->ia], m.list, m.w, m.wb, norm_of(m), cnt));
+                                                     c->U[c->ia], c->rlist, m.w, m.wb, norm_of(m), cnt));
   else
     LAUNCH_TIMED(c, "relax_list", hipLaunchKernelGGL(relax_list_k<false>, dim3(grid), dim3(256), 0, c->stream, c->p,
-                                                     c->U[c->ia], m.list, m.w, m.wb, norm_of(m), cnt));
+                                                     c->U[c->ia], c->rlist, m.w, m.wb, norm_of(m), cnt));
   return FK_OK;
 }
 
@@ -667,7 +669,7 @@ int fk_relax_emit_impl(fk_ctx *c, bool *done) {
 
 void fk_mlp_free(fk_ctx *c) {
   Mlp &m = c->mlp;
-  void *ptrs[] = {m.w, m.m, m.v, m.wb, m.grad, m.part, m.dx, m.dy, m.list};
+  void *ptrs[] = {m.w, m.m, m.v, m.wb, m.grad, m.part, m.dx, m.dy};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   m = Mlp{};

@@ -11,18 +11,18 @@ namespace fk {
 
 // ---------------- phase relaxation (relaxation.cu, thermRlx.cu, chemRlx.cu) ----------------
 
-__device__ inline double clampd(const fk_params &P, double a) {
+FK_HD inline double clampd(const fk_params &P, double a) {
   return fmin(fmax(P.avfMin / 100, a), 1.0 - P.avfMin / 100);
 }
-__device__ inline double eos_p(const fk_params &P, int k, double rho, double e) {
+FK_HD inline double eos_p(const fk_params &P, int k, double rho, double e) {
   return (P.gamma[k] - 1) * rho * e - P.gamma[k] * P.pInf[k] - (P.gamma[k] - 1) * P.eta[k] * rho;
 }
-__device__ inline double eos_T(const fk_params &P, int k, double rho, double e) {
+FK_HD inline double eos_T(const fk_params &P, int k, double rho, double e) {
   double p = (P.gamma[k] - 1) * rho * e - P.gamma[k] * P.pInf[k] - (P.gamma[k] - 1) * P.eta[k] * rho;
   return (p + P.pInf[k]) / ((P.Cp[k] / P.gamma[k]) * (P.gamma[k] - 1) * rho);
 }
 
-__device__ inline double tsat_dev(const fk_params &P, double Pr, double temp, double urf, int *iters) {
+FK_HD inline double tsat_dev(const fk_params &P, double Pr, double temp, double urf, int *iters) {
   const double Pinf = P.pInf[0], g1 = P.gamma[0], g2 = P.gamma[1], Cp1 = P.Cp[0], Cp2 = P.Cp[1];
   const double Cv1 = Cp1 / g1, Cv2 = Cp2 / g2;
   const double et2 = P.satConst[0] * fkmath::log10(Pr) + P.satConst[1];
@@ -31,11 +31,13 @@ __device__ inline double tsat_dev(const fk_params &P, double Pr, double temp, do
   const double Cs = (Cp2 - Cp1) / (Cp2 - Cv2);
   const double Ds = (Cp1 - Cv1) / (Cp2 - Cv2);
   const double dt = 1.e-3;
+  // loop-invariant logarithms, evaluated once (same values in every iteration)
+  const double lpp = fkmath::log10(Pr + Pinf), lp = fkmath::log10(Pr);
   double fun = 1, T = temp;
   int itr = 0;
   while (fabs(fun) > 1.e-2 && itr < 50) {
-    fun = As + Bs / T + Cs * fkmath::log10(T) + Ds * fkmath::log10(Pr + Pinf) - fkmath::log10(Pr);
-    double fd = As + Bs / (T + dt) + Cs * fkmath::log10(T + dt) + Ds * fkmath::log10(Pr + Pinf) - fkmath::log10(Pr);
+    fun = As + Bs / T + Cs * fkmath::log10(T) + Ds * lpp - lp;
+    double fd = As + Bs / (T + dt) + Cs * fkmath::log10(T + dt) + Ds * lpp - lp;
     double gr = (fd - fun) / dt;
     T = T - urf * fun / gr;
     itr++;
@@ -47,7 +49,7 @@ __device__ inline double tsat_dev(const fk_params &P, double Pr, double temp, do
 struct ThermSys {
   int pm;
   double ar[2], enr[2];
-  __device__ void res(const fk_params &P, double x[4], double *r0, double *r1) const {
+  FK_HD void res(const fk_params &P, double x[4], double *r0, double *r1) const {
     const int k = 1 - pm;
     double s1 = ar[k] / x[k];
     double s2 = ar[k] * (x[2 + k] - enr[k]);
@@ -60,7 +62,7 @@ struct ThermSys {
   }
 };
 
-__device__ inline int therm_dev(const fk_params &P, double *U) {
+FK_HD inline int therm_dev(const fk_params &P, double *U) {
   const int itrMax = P.thermItrMax;
   const double urf = P.thermUrf;
   double avf0 = clampd(P, U[0]);
@@ -103,7 +105,7 @@ __device__ inline int therm_dev(const fk_params &P, double *U) {
     for (int i = 0; i < 2; i++)
       for (int j = 0; j < 2; j++) {
         J[i][j] = (rd[i][j] - r[i]) / e[j];
-        if (isnan(J[i][j])) abortFlag = 1;
+        if (__builtin_isnan(J[i][j])) abortFlag = 1;
       }
     const double det = J[0][0] * J[1][1] - J[1][0] * J[0][1];
     double I00 = J[1][1] / det, I11 = J[0][0] / det, I10 = -J[1][0] / det, I01 = -J[0][1] / det;
@@ -141,7 +143,7 @@ __device__ inline int therm_dev(const fk_params &P, double *U) {
 
 // Gibbs energy of chemRlx.cu; log10(T^g / (p + pinf)^(g - 1)) taken as
 // g log10(T) - (g - 1) log10(p + pinf).
-__device__ inline double gibbs_dev(const fk_params &P, int k, double pr, double T) {
+FK_HD inline double gibbs_dev(const fk_params &P, int k, double pr, double T) {
   const double et = k == 0 ? 0.0 : P.satConst[0] * fkmath::log10(pr) + P.satConst[1];
   const double h = P.Cp[k] * T + P.eta[k];
   const double s = (P.Cp[k] / P.gamma[k]) * (P.gamma[k] * fkmath::log10(T) - (P.gamma[k] - 1) * fkmath::log10(pr + P.pInf[k])) + et;
@@ -151,7 +153,7 @@ __device__ inline double gibbs_dev(const fk_params &P, int k, double pr, double 
 struct ChemSys {
   int pm;
   double rhoMix, rEint;
-  __device__ void res(const fk_params &P, double x[6], double *r0, double *r1, double *r2) const {
+  FK_HD void res(const fk_params &P, double x[6], double *r0, double *r1, double *r2) const {
     const int k = 1 - pm;
     const double s1 = x[k], s2 = x[k] * x[k + 2], s3 = x[k] * x[k + 2] * x[k + 4];
     x[pm] = 1 - s1;
@@ -169,7 +171,7 @@ struct ChemSys {
   }
 };
 
-__device__ inline void chk_update(const fk_params &P, double xOld, double *x, double dx, int i, float urf, int *abortFlag,
+FK_HD inline void chk_update(const fk_params &P, double xOld, double *x, double dx, int i, float urf, int *abortFlag,
                            double rho) {
   if (i == 0 || i == 1) {
     if (x[i] < 0)
@@ -202,7 +204,7 @@ __device__ inline void chk_update(const fk_params &P, double xOld, double *x, do
   }
 }
 
-__device__ inline int chem_dev(const fk_params &P, double *U) {
+FK_HD inline int chem_dev(const fk_params &P, double *U) {
   const int itrMax = P.chemItrMax;
   const double urf = P.chemUrf;
   double x[6], ax[6];
@@ -245,7 +247,7 @@ __device__ inline int chem_dev(const fk_params &P, double *U) {
     for (int i = 0; i < 3; i++)
       for (int j = 0; j < 3; j++) {
         J[i][j] = (rd[i][j] - r[i]) / e[j];
-        if (isnan(J[i][j])) abortFlag = 1;
+        if (__builtin_isnan(J[i][j])) abortFlag = 1;
       }
     const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
                        J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
@@ -301,7 +303,7 @@ __device__ inline int chem_dev(const fk_params &P, double *U) {
 // dT f = (C T ln10 - B) / (T^2 ln10) > 0 (B <= 0, C >= 0, checked on the
 // host) f is increasing in T, so T_L > Tsat(p_L) <=> f(T_L; p_L) > 0: the
 // classification of partitionRelaxation without the Tsat iteration.
-__device__ inline double sat_fun(const fk_params &P, double Pr, double T) {
+FK_HD inline double sat_fun(const fk_params &P, double Pr, double T) {
   const double g1 = P.gamma[0], g2 = P.gamma[1], Cp1 = P.Cp[0], Cp2 = P.Cp[1];
   const double Cv1 = Cp1 / g1, Cv2 = Cp2 / g2;
   const double et2 = P.satConst[0] * fkmath::log10(Pr) + P.satConst[1];
@@ -310,6 +312,25 @@ __device__ inline double sat_fun(const fk_params &P, double Pr, double T) {
   const double Cs = (Cp2 - Cp1) / (Cp2 - Cv2);
   const double Ds = (Cp1 - Cv1) / (Cp2 - Cv2);
   return As + Bs / T + Cs * fkmath::log10(T) + Ds * fkmath::log10(Pr + P.pInf[0]) - fkmath::log10(Pr);
+}
+
+// Sign of f(T; p) (> 0: superheated liquid), evaluated in FP32 first and in
+// FP64 only when |f| < 1e-3: the FP32 terms (|term| < 100) carry errors
+// below 1e-4, and df/dT ~ 1e-2 / K, so only states within ~0.1 K of
+// saturation need the FP64 evaluation.
+FK_HD inline bool sat_superheated(const fk_params &P, double Pr, double T) {
+  const double g1 = P.gamma[0], g2 = P.gamma[1], Cp1 = P.Cp[0], Cp2 = P.Cp[1];
+  const double Cv1 = Cp1 / g1, Cv2 = Cp2 / g2;
+  const double iden = 1 / (Cp2 - Cv2);
+  const float pr = (float)Pr, tf = (float)T;
+  const float et2 = (float)P.satConst[0] * log10f(pr) + (float)P.satConst[1];
+  const float As = (float)(Cp1 - Cp2) * (float)iden + et2 * (float)iden;
+  const float Bs = (float)((P.eta[0] - P.eta[1]) * iden);
+  const float Cs = (float)((Cp2 - Cp1) * iden);
+  const float Ds = (float)((Cp1 - Cv1) * iden);
+  const float f = As + Bs / tf + Cs * log10f(tf) + Ds * log10f(pr + (float)P.pInf[0]) - log10f(pr);
+  if (fabsf(f) > 1e-3f && __builtin_isfinite(f)) return f > 0;
+  return sat_fun(P, Pr, T) > 0;
 }
 
 // Thermal (p-T) equilibrium problem of one cell as thermalRlx sets it up
@@ -323,17 +344,16 @@ struct ThermProb {
   int pm, k;
 };
 
-__device__ inline void therm_setup(const fk_params &P, const double *U, ThermProb &t) {
+FK_HD inline void therm_setup(const fk_params &P, const double *U, ThermProb &t) {
   const double a0 = clampd(P, U[0]), a1 = 1 - a0;
-  const double r0 = U[1] / a0, r1 = U[2] / a1;
-  t.ar[0] = a0 * r0;
-  t.ar[1] = a1 * r1;
-  t.enr[0] = U[6] / t.ar[0];
-  t.enr[1] = U[7] / t.ar[1];
-  const double rhoMix = t.ar[0] + t.ar[1];
-  const double v0 = U[3] / rhoMix, v1 = U[4] / rhoMix, v2 = U[5] / rhoMix;
-  const double de = U[8] / rhoMix - (t.ar[0] / rhoMix) * t.enr[0] - (t.ar[1] / rhoMix) * t.enr[1] -
-                    0.5 * (v0 * v0 + v1 * v1 + v2 * v2);
+  t.ar[0] = U[1];
+  t.ar[1] = U[2];
+  t.enr[0] = U[6] / U[1];
+  t.enr[1] = U[7] / U[2];
+  const double ir = 1 / (U[1] + U[2]);
+  const double ke = 0.5 * (U[3] * U[3] + U[4] * U[4] + U[5] * U[5]) * ir * ir;
+  // de = rho E / rho - Y_L e_L - Y_G e_G - |u|^2 / 2 (Y_k e_k = U[6 + k] / rho)
+  const double de = (U[8] - U[6] - U[7]) * ir - ke;
   t.enr[0] += de;
   t.enr[1] += de;
   t.pm = (a1 > a0) ? 1 : 0;
@@ -341,7 +361,7 @@ __device__ inline void therm_setup(const fk_params &P, const double *U, ThermPro
 }
 
 // Cold start of the thermal problem: the cell's own (rho_k, e_k + de).
-__device__ inline void therm_cold(const fk_params &P, const double *U, const ThermProb &t, double &xr, double &xe) {
+FK_HD inline void therm_cold(const fk_params &P, const double *U, const ThermProb &t, double &xr, double &xe) {
   const double a0 = clampd(P, U[0]);
   xr = t.k == 0 ? U[1] / a0 : U[2] / (1 - a0);
   xe = t.enr[t.k];
@@ -351,34 +371,47 @@ __device__ inline void therm_cold(const fk_params &P, const double *U, const The
 // until |d rho_k| <= tol |rho_k| and |d e_k| <= tol |e_k|.  Returns the
 // iterations (> 0), or -1 (no convergence within itmax, singular Jacobian,
 // or an iterate outside rho_k > ar_k); xr, xe hold the last iterate.
-__device__ inline int therm_newton(const fk_params &P, const ThermProb &t, double &xr, double &xe, double tol,
-                                   int itmax) {
+// Residuals p_k - p_m and T_k - T_m (signs as thermRlx.cu:95-103) with
+//   p = (g - 1) rho (e - eta) - g pinf,  T = (e - eta - pinf / rho) / Cv,
+// the stiffened-gas EOS of eqos.h rearranged so that an iteration takes
+// three divisions.
+FK_HD inline int therm_newton(const fk_params &P, const ThermProb &t, double &xr, double &xe, double tol,
+                              int itmax) {
   const int pm = t.pm, k = t.k;
-  if (!(xr > t.ar[k]) || !isfinite(xe)) return -1;
+  if (!(xr > t.ar[k]) || !__builtin_isfinite(xe)) return -1;
   const double sk = k == 0 ? 1.0 : -1.0;  // sign of phase k in r = q_0 - q_1
   const double gk = P.gamma[k], gm = P.gamma[pm];
-  const double cvk = P.Cp[k] / gk, cvm = P.Cp[pm] / gm;
+  const double ick = gk / P.Cp[k], icm = gm / P.Cp[pm];  // 1 / Cv
+  const double ark = t.ar[k], iarm = 1 / t.ar[pm];
+  const double ratio = ark * iarm;  // d e_m / d e_k = -ratio
+  const double ek0 = t.enr[k], em0 = t.enr[pm];
+  const double etk = P.eta[k], etm = P.eta[pm], pik = P.pInf[k], pim = P.pInf[pm];
   for (int it = 1; it <= itmax; ++it) {
-    const double rm = t.ar[pm] / (1 - t.ar[k] / xr);
-    const double em = t.enr[pm] - (t.ar[k] / t.ar[pm]) * (xe - t.enr[k]);
+    const double ixr = 1 / xr;
+    const double q = ark * ixr;  // alpha_k
+    const double omq = 1 - q;
+    const double rm = t.ar[pm] / omq;
+    const double irm = omq * iarm;
+    const double em = em0 - ratio * (xe - ek0);
     if (!(rm > 0)) return -1;
-    const double pk = eos_p(P, k, xr, xe), pmm = eos_p(P, pm, rm, em);
-    const double tk = eos_T(P, k, xr, xe), tm = eos_T(P, pm, rm, em);
+    const double pk = (gk - 1) * xr * (xe - etk) - gk * pik;
+    const double pmm = (gm - 1) * rm * (em - etm) - gm * pim;
+    const double tk = (xe - etk - pik * ixr) * ick;
+    const double tm = (em - etm - pim * irm) * icm;
     const double f0 = sk * (pk - pmm), f1 = sk * (tk - tm);
-    // d rho_m / d rho_k and d e_m / d e_k of the closure
-    const double drm = -(rm * rm / t.ar[pm]) * (t.ar[k] / (xr * xr));
-    const double dem = -t.ar[k] / t.ar[pm];
-    const double J00 = sk * ((gk - 1) * (xe - P.eta[k]) - (gm - 1) * (em - P.eta[pm]) * drm);
-    const double J01 = sk * ((gk - 1) * xr - (gm - 1) * rm * dem);
-    const double J10 = sk * (P.pInf[k] / (cvk * xr * xr) - P.pInf[pm] / (cvm * rm * rm) * drm);
-    const double J11 = sk * (1.0 / cvk - dem / cvm);
+    const double drm = -rm * rm * iarm * q * ixr;  // d rho_m / d rho_k
+    const double J00 = sk * ((gk - 1) * (xe - etk) - (gm - 1) * (em - etm) * drm);
+    const double J01 = sk * ((gk - 1) * xr + (gm - 1) * rm * ratio);
+    const double J10 = sk * (pik * ick * ixr * ixr - pim * icm * irm * irm * drm);
+    const double J11 = sk * (ick + ratio * icm);
     const double det = J00 * J11 - J01 * J10;
-    if (!(fabs(det) > 0) || !isfinite(det)) return -1;
-    const double dr = (J11 * f0 - J01 * f1) / det;
-    const double dee = (J00 * f1 - J10 * f0) / det;
+    if (!(fabs(det) > 0) || !__builtin_isfinite(det)) return -1;
+    const double idet = 1 / det;
+    const double dr = (J11 * f0 - J01 * f1) * idet;
+    const double dee = (J00 * f1 - J10 * f0) * idet;
     xr -= dr;
     xe -= dee;
-    if (!(xr > t.ar[k]) || !isfinite(xe)) return -1;
+    if (!(xr > t.ar[k]) || !__builtin_isfinite(xe)) return -1;
     if (fabs(dr) <= tol * fabs(xr) && fabs(dee) <= tol * fabs(xe)) return it;
   }
   return -1;
@@ -386,7 +419,7 @@ __device__ inline int therm_newton(const fk_params &P, const ThermProb &t, doubl
 
 // Writes alpha_L, alpha_L rho_L e_L, alpha_G rho_G e_G of the solution
 // (updateConsVec, thermRlx.cu:172-187).
-__device__ inline void therm_commit(const fk_params &P, const ThermProb &t, double xr, double xe, double *U) {
+FK_HD inline void therm_commit(const fk_params &P, const ThermProb &t, double xr, double xe, double *U) {
   const int pm = t.pm, k = t.k;
   const double s1 = t.ar[k] / xr, s2 = t.ar[k] * (xe - t.enr[k]);
   double x[4];
@@ -401,7 +434,7 @@ __device__ inline void therm_commit(const fk_params &P, const ThermProb &t, doub
 
 // Thermal equilibrium by Newton polish from the warm start (rho_k, e_k).
 // Returns iterations (> 0) or -1 (the caller falls back).
-__device__ inline int therm_polish(const fk_params &P, double *U, double rk0, double ek0, double tol, int itmax) {
+FK_HD inline int therm_polish(const fk_params &P, double *U, double rk0, double ek0, double tol, int itmax) {
   ThermProb t;
   therm_setup(P, U, t);
   double xr = rk0, xe = ek0;
@@ -414,7 +447,7 @@ __device__ inline int therm_polish(const fk_params &P, double *U, double rk0, do
 // Newton with a forward-difference Jacobian (step 1e-7 |x|) and full steps,
 // from the warm start (alpha_k, rho_k, e_k), to |dx_i| <= tol |x_i|.
 // Returns iterations or -1 (out of bounds, singular, not converged).
-__device__ inline int chem_polish(const fk_params &P, double *U, double ak0, double rk0, double ek0, double tol,
+FK_HD inline int chem_polish(const fk_params &P, double *U, double ak0, double rk0, double ek0, double tol,
                                   int itmax) {
   double x0[6];
   x0[0] = clampd(P, U[0]);
@@ -446,7 +479,7 @@ __device__ inline int chem_polish(const fk_params &P, double *U, double ak0, dou
       for (int j = 0; j < 3; ++j) J[i][j] = (rd[i][j] - r[i]) / e[j];
     const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
                        J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-    if (!(fabs(det) > 0) || !isfinite(det)) return -1;
+    if (!(fabs(det) > 0) || !__builtin_isfinite(det)) return -1;
     // Cramer's rule for J dx = r
     double dx[3];
     for (int c = 0; c < 3; ++c) {
@@ -463,7 +496,7 @@ __device__ inline int chem_polish(const fk_params &P, double *U, double ak0, dou
       if (!(fabs(dx[q]) <= tol * fabs(x[j]))) small = false;
     }
     // bounds of checkUpdate (chemRlx.cu:139-162)
-    if (!(x[k] > 0 && x[k] < 1 && x[k + 2] > 0 && x[k] * x[k + 2] <= ch.rhoMix) || !isfinite(x[k + 4])) return -1;
+    if (!(x[k] > 0 && x[k] < 1 && x[k + 2] > 0 && x[k] * x[k + 2] <= ch.rhoMix) || !__builtin_isfinite(x[k + 4])) return -1;
     if (small) {
       const double s1 = x[k], s2 = x[k] * x[k + 2], s3 = x[k] * x[k + 2] * x[k + 4];
       x[pm] = 1 - s1;
@@ -485,7 +518,7 @@ __device__ inline int chem_polish(const fk_params &P, double *U, double ak0, dou
 // Surrogate features of a conserved state: (log rho, e, Y_L) with e the
 // mixture internal energy per unit mass and Y_L the liquid mass fraction.
 // (rho, e, Y_L) fix the thermal equilibrium; (rho, e) fix the chemical one.
-__device__ inline void mlp_features(const fk_params &P, const double *U, float *x) {
+FK_HD inline void mlp_features(const fk_params &P, const double *U, float *x) {
   (void)P;
   const double rho = U[1] + U[2];
   const double ke = 0.5 * (U[3] * U[3] + U[4] * U[4] + U[5] * U[5]) / rho;
@@ -494,13 +527,22 @@ __device__ inline void mlp_features(const fk_params &P, const double *U, float *
   x[2] = (float)(U[1] / rho);
 }
 
-// Targets of a relaxed state: (T_L, asinh(p_L / 1e4), alpha_L).
-__device__ inline void mlp_targets(const fk_params &P, const double *U, float *y) {
+// Targets of a relaxed state: (T_L, asinh(p_L / 1e4), alpha_L), or NaN when
+// the state is not at p-T equilibrium (|p_L - p_G| > 1e-6 max(|p_L|, 1e3) Pa
+// or |T_L - T_G| > 1e-3 K): the plain solve stops at loose absolute
+// tolerances or aborts to its best iterate, and such labels are not a
+// function of the features.  Returns whether the targets are valid.
+FK_HD inline bool mlp_targets(const fk_params &P, const double *U, float *y) {
   const double a = clampd(P, U[0]);
-  const double rl = U[1] / a, el = U[6] / (a * rl);
-  y[0] = (float)eos_T(P, 0, rl, el);
-  y[1] = (float)asinh(eos_p(P, 0, rl, el) / 1e4);
-  y[2] = (float)a;
+  const double rl = U[1] / a, el = U[6] / U[1];
+  const double rg = U[2] / (1 - a), eg = U[7] / U[2];
+  const double pl = eos_p(P, 0, rl, el), pg = eos_p(P, 1, rg, eg);
+  const double tl = eos_T(P, 0, rl, el), tg = eos_T(P, 1, rg, eg);
+  const bool ok = fabs(pl - pg) <= 1e-6 * fmax(fabs(pl), 1e3) && fabs(tl - tg) <= 1e-3;
+  y[0] = ok ? (float)tl : NAN;
+  y[1] = ok ? (float)asinh(pl / 1e4) : NAN;
+  y[2] = ok ? (float)a : NAN;
+  return ok;
 }
 
 }  // namespace fk

@@ -45,6 +45,13 @@ def fit_norm(x, y):
     return np.concatenate([x.mean(0), xs, y.mean(0), ys]).astype(np.float32)
 
 
+def finite(x, y):
+    """Rows with finite features and targets (the labelling marks states that
+    are not chemical-class or did not reach equilibrium with NaN targets)."""
+    ok = np.isfinite(x).all(1) & np.isfinite(y).all(1)
+    return x[ok], y[ok]
+
+
 def uniform_states(n, params, seed=0, rho_lo=0.02, rho_hi=1000.0):
     """Pre-relaxation two-phase states with mixture density uniform in
     [rho_lo, rho_hi]: phasic pressures drawn log-uniformly in [1e3, 5e7] Pa
@@ -175,11 +182,9 @@ def train_on_solver(solver, case_U, params, n_solver=4_000_000, n_uniform=1_000_
     sg.emit(0, 0)
     xs, ys = sg.data()
     xu, yu = sg.label(uniform_states(n_uniform, params, seed=seed + 1))
-    log(f"surrogate: {len(xs)} solver states (stride {stride}) + {len(xu)} uniform-density states")
-    x = np.concatenate([xs, xu])
-    y = np.concatenate([ys, yu])
-    ok = np.isfinite(x).all(1) & np.isfinite(y).all(1)
-    x, y = x[ok], y[ok]
+    log(f"surrogate: {len(xs)} solver states (stride {stride}) + {len(xu)} uniform-density states, "
+        f"{int(np.isfinite(ys).all(1).sum() + np.isfinite(yu).all(1).sum())} chemical-class equilibrium labels")
+    x, y = finite(np.concatenate([xs, xu]), np.concatenate([ys, yu]))
     norm = fit_norm(x, y)
     sg.set_data(x, y)
     sg.set_weights(init_weights(seed), norm)

@@ -401,9 +401,10 @@ double tsat(const fk_params &p, double P, double temp, double urf, int *iters) {
   double Ds = (Cp1 - Cv1) / (Cp2 - Cv2);
   double dt = 1.e-3, fun = 1, T = temp;
   int itr = 0;
+  const double lpp = fkmath::log10(P + Pinf), lp = fkmath::log10(P);  // loop invariants
   while (std::fabs(fun) > 1.e-2 && itr < 50) {
-    fun = As + Bs / T + Cs * fkmath::log10(T) + Ds * fkmath::log10(P + Pinf) - fkmath::log10(P);
-    double fun_dt = As + Bs / (T + dt) + Cs * fkmath::log10(T + dt) + Ds * fkmath::log10(P + Pinf) - fkmath::log10(P);
+    fun = As + Bs / T + Cs * fkmath::log10(T) + Ds * lpp - lp;
+    double fun_dt = As + Bs / (T + dt) + Cs * fkmath::log10(T + dt) + Ds * lpp - lp;
     double grad_f = (fun_dt - fun) / dt;
     T = T - urf * fun / grad_f;
     itr++;
@@ -1064,13 +1065,6 @@ int orc_compute_rhs(orc *o) {
       double cont[MAXEQ];
       side_contrib(p, o->ff[(size_t)f], nv, A, Uc, side, cont);
       for (int v = 0; v < neq; ++v) r[v] += cont[v];
-      if (visc) {
-        const double *vv = &o->visc[(size_t)(f * 5)];
-        for (int i = 0; i < 3; ++i) r[iu + i] += side == 0 ? vv[i] * A : -(vv[i]) * A;
-        int ie = single ? 4 : 8;
-        r[ie] += side == 0 ? vv[3] * A : -(vv[3]) * A;
-        if (single && p.heatFlux == 1) r[4] += side == 0 ? vv[4] * A : -(vv[4]) * A;
-      }
       // computeFaceMaxSignal for both face states
       int bc = o->bc[(size_t)f];
       double cm[2] = {0.0, 0.0};
@@ -1098,6 +1092,20 @@ int orc_compute_rhs(orc *o) {
       if (!(sig >= cMax)) sig = cMax;  // atomicMaxDouble
     }
     o->signal[(size_t)c] = sig;
+    // viscous contributions after the inviscid ones, again in ascending face
+    // order (the reference adds both with atomics, in no fixed order)
+    if (visc) {
+      for (int k = 0; k < 4; ++k) {
+        int64_t f = o->cf[(size_t)(4 * c + k)];
+        int side = o->fc2el[(size_t)(2 * f)] == c ? 0 : 1;
+        double A = o->area[(size_t)f];
+        const double *vv = &o->visc[(size_t)(f * 5)];
+        for (int i = 0; i < 3; ++i) r[iu + i] += side == 0 ? vv[i] * A : -(vv[i]) * A;
+        int ie = single ? 4 : 8;
+        r[ie] += side == 0 ? vv[3] * A : -(vv[3]) * A;
+        if (single && p.heatFlux == 1) r[4] += side == 0 ? vv[4] * A : -(vv[4]) * A;
+      }
+    }
   }
   return FK_OK;
 }
