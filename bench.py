@@ -62,13 +62,15 @@ def kernel_bytes(name, B, neq):
     }.get(name)
 
 
-def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None, fuse=0):
+def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None, fuse=0, stage_f=1, stage_g=0):
     """Time `steps` explicitLoop steps on the device (state resident in HBM).
     train: dict of surrogate-training options (done before the timed region)."""
     from fooskagpu_amd import GpuSolver, surrogate
     p = dict(case.params, precision=precision, relaxMode=0)
     g = GpuSolver(p, case.mesh, case.U, device=device)
     g.set_tuning("fuse_update", fuse)
+    g.set_tuning("stage_f", stage_f)
+    g.set_tuning("stage_g", stage_g)
     info = {}
     if train is not None:
         t0 = time.perf_counter()
@@ -132,6 +134,10 @@ def main():
     ap.add_argument("--no-mlp", action="store_true", help="skip the surrogate run")
     ap.add_argument("--order", default="morton", choices=["rcm", "morton", "none"], help="cell order of the mesh")
     ap.add_argument("--train-steps", type=int, default=3000)
+    ap.add_argument("--stage-g", type=int, default=0, choices=[0, 1],
+                    help="pass G stages neighbour primitive states in LDS; same results")
+    ap.add_argument("--stage-f", type=int, default=1, choices=[0, 1],
+                    help="pass F stages the workgroup's face states in LDS (mixed precision); same results")
     ap.add_argument("--fuse", type=int, default=0, choices=[0, 1],
                     help="RK update inside pass F (1) or residual + separate update kernel (0); same results")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30,
@@ -166,7 +172,8 @@ def main():
         if (name == "fp64" and args.no_fp64) or (name == "mixed_mlp" and args.no_mlp):
             continue
         log(f"{name}: warmup {args.warmup}, timing {args.steps} steps")
-        el, kt, U, dt, info = run_gpu(case, prec, args.steps, args.warmup, relax_mode=mode, train=tr, fuse=args.fuse)
+        el, kt, U, dt, info = run_gpu(case, prec, args.steps, args.warmup, relax_mode=mode, train=tr, fuse=args.fuse,
+                                      stage_f=args.stage_f, stage_g=args.stage_g)
         rate = C * s * args.steps / el
         log(f"{name}: {el:.3f} s, {rate:.4g} cell-stage/s, last dt {dt:.3e}, finite={bool(np.isfinite(U).all())}")
         for k, (n, ms) in sorted(kt.items(), key=lambda kv: -kv[1][1]):
@@ -202,7 +209,8 @@ def main():
                                f"WALE, SSP-RK3, CFL 0.1, mixed precision, "
                                + ("MLP warm start + FP64 Newton polish" if head == "mixed_mlp" else
                                   "plain Newton relaxation"),
-                   "cells": C, "faces": case.mesh.n_faces, "cell_order": args.order, "fuse_update": args.fuse},
+                   "cells": C, "faces": case.mesh.n_faces, "cell_order": args.order, "fuse_update": args.fuse,
+                   "stage_f": args.stage_f, "stage_g": args.stage_g},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": None,
                      "kernel": dom, "bytes_per_cell": bpc, "avg_launch_ms": avg_f * 1e3},

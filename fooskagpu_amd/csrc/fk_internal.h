@@ -85,11 +85,14 @@ struct fk_ctx {
   int64_t relax_calls = 0;  // relaxations summed in lcount[8, 13)
   // relaxation work list (chemical cells / surrogate list), one entry per cell
   uint32_t *rlist = nullptr;
+  unsigned long long *rpart = nullptr;  // per-workgroup relaxation counts [blocks][4]
   int64_t rlist_cap = 0;
   double dt = 0, time = 0;
   int64_t step = 0;
   bool rhs_valid = false;
   bool fuse_update = false;  // RK update inside pass F (else residual + rk_update)
+  bool stage_f = true;       // pass F stages the workgroup's face states in LDS (mixed only)
+  bool stage_g = false;      // pass G stages neighbour primitive states in LDS
   fkx::Timer tm;
   fkx::Mlp mlp;
   // host geometry kept to rebuild the precision-dependent arrays

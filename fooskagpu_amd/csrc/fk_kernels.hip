@@ -268,14 +268,135 @@ __global__ void __launch_bounds__(kBlock) pass_g(fk_params P, int64_t nc, const 
   }
 }
 
+// Pass G without LDS staging: every cell converts its 4 neighbours itself
+// (5 cons2prim per cell) and keeps all variables in registers; full face
+// records are written once.  Same arithmetic and order as pass_g.
+template <int NEQ, typename T, bool VISC, bool HEAT>
+__global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc, const double *__restrict__ U,
+                                                 const int32_t *__restrict__ nbr, const int32_t *__restrict__ mir,
+                                                 const void *__restrict__ geo, const void *__restrict__ rvec,
+                                                 const double *__restrict__ vol, T *__restrict__ face,
+                                                 T *__restrict__ gvel, double *__restrict__ dgrad,
+                                                 double *__restrict__ dtheta) {
+  const int64_t c = xcd_block() * kBlock + threadIdx.x;
+  if (c >= nc) return;
+  const fk::Mat<double> md = fk::make_mat<double>(P);
+  const fk::Mat<T> mt = fk::make_mat<T>(P);
+  double Wd[NEQ];
+  fk::cons2prim<NEQ>(md, U + c * NEQ, Wd);
+  T W[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) W[v] = (T)Wd[v];
+  T g[NEQ][3], mn[NEQ], mx[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) {
+    g[v][0] = g[v][1] = g[v][2] = 0;
+    mn[v] = (T)1e12;
+    mx[v] = (T)-1e12;
+  }
+  const T V = (T)vol[c];
+#pragma unroll 1
+  for (int j = 0; j < 4; ++j) {
+    const int64_t s = c * 4 + j;
+    const int nb = nbr[s];
+    const bool own = mir[s] & 1;
+    const V4<T> ga = ld4<T>(geo, s);
+    const T n[3] = {ga.x, ga.y, ga.z};
+    T Wn[NEQ];
+    if (nb >= 0) {
+      double t[NEQ];
+      fk::cons2prim<NEQ>(md, U + (int64_t)nb * NEQ, t);
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) Wn[v] = (T)t[v];
+    } else {
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) Wn[v] = W[v];
+      fk::ghost<NEQ, T>(mt, -nb, n, Wn);
+    }
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) {
+      const T o = Wn[v];
+      if (!(mx[v] >= o)) mx[v] = o;
+      if (!(mn[v] <= o)) mn[v] = o;
+      const T avg = (W[v] + Wn[v]) / 2;
+      if (own) {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) g[v][d] += avg * n[d] * ga.w / V;
+      } else {
+#pragma unroll
+        for (int d = 0; d < 3; ++d) g[v][d] += -avg * n[d] * ga.w / V;
+      }
+    }
+  }
+  // limiter: theta = min over faces of clamp(phi, 0, 1)
+  T th[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) th[v] = (T)1e9;
+  V4<T> rr[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) rr[j] = ld4<T>(rvec, c * 4 + j);
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) {
+      const T wu = W[v] + rr[j].x * g[v][0] + rr[j].y * g[v][1] + rr[j].z * g[v][2];
+      T phi;
+      if (wu - W[v] != 0)
+        phi = (wu > W[v]) ? (mx[v] - W[v]) / (2 * (wu - W[v])) : (mn[v] - W[v]) / (2 * (wu - W[v]));
+      else
+        phi = 1;
+      const T tt = fmax((T)0, fmin(phi, (T)1));
+      if (!(th[v] <= tt)) th[v] = tt;
+    }
+  }
+  // MUSCL face states
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    T *out = face + (c * 4 + j) * NEQ;
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) {
+      const T ext = rr[j].x * g[v][0] + rr[j].y * g[v][1] + rr[j].z * g[v][2];
+      out[v] = W[v] + th[v] * ext;
+    }
+  }
+  if constexpr (VISC) {
+    constexpr int iu = Model<NEQ>::iu;
+    constexpr int NG = ngrad<NEQ, HEAT>();
+    T *G = gvel + c * NG;
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+      for (int d = 0; d < 3; ++d) G[i * 3 + d] = g[iu + i][d];
+    if constexpr (HEAT) {
+      const T gm = (T)P.gamma[0], pi = (T)P.pInf[0];
+      const T R = (T)P.Cp[0] * (gm - 1) / gm;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) G[9 + d] = (g[4][d] - ((W[4] + pi) / W[0]) * g[0][d]) / (W[0] * R);
+    }
+  }
+  if (dgrad) {
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) {
+      dtheta[c * NEQ + v] = (double)th[v];
+#pragma unroll
+      for (int d = 0; d < 3; ++d) dgrad[(c * NEQ + v) * 3 + d] = (double)g[v][d];
+    }
+  }
+}
+
 // Pass F: Riemann problem on each face of the cell, non-conservative terms,
 // pressure correction, viscous/WALE terms, per-cell gather and (MODE 0) the
 // RK stage, (MODE 1) residual + signal speed for the time step, or (MODE 2)
 // the residual only.
 // (computeRHS.cu:78-146, computeViscousStresses.cu:66-174, update.cu:3-103,
 // computeTimeStep.cu:97-159)
-template <int NEQ, typename T, bool VISC, bool HEAT, int MODE>
-__global__ void __launch_bounds__(kBlock, (MODE == 1 ? 2 : 3)) pass_f(fk_params P, int64_t nc, const double *__restrict__ U,
+//
+// STAGE: the workgroup first copies the face states of its own cells (4 slots
+// each) and their velocity gradients into LDS; the mirror states and
+// neighbour gradients of faces inside the workgroup are then read from LDS,
+// only those of faces to other workgroups from global memory.
+template <int NEQ, typename T, bool VISC, bool HEAT, int MODE, bool STAGE>
+__global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3)) pass_f(fk_params P, int64_t nc, const double *__restrict__ U,
                                                  const double *__restrict__ U0, double *__restrict__ Unew,
                                                  const int32_t *__restrict__ nbr, const int32_t *__restrict__ mir,
                                                  const void *__restrict__ geo, const T *__restrict__ wls,
@@ -283,8 +404,35 @@ __global__ void __launch_bounds__(kBlock, (MODE == 1 ? 2 : 3)) pass_f(fk_params 
                                                  const T *__restrict__ face, const T *__restrict__ gvel,
                                                  double *__restrict__ rhs_out, double *__restrict__ dtc, int irk,
                                                  double dt_val, const double *__restrict__ dt_ptr) {
-  const int64_t c = xcd_block() * kBlock + threadIdx.x;
+  constexpr int NGS = ngrad<NEQ, HEAT>();
+  const int64_t c0 = xcd_block() * kBlock;
+  const int64_t c = c0 + threadIdx.x;
+  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
+  T *sF = reinterpret_cast<T *>(lds_raw);
+  T *sG = sF + kBlock * 4 * NEQ;
+  if constexpr (STAGE) {
+    if (c < nc) {
+      const T *src = face + c * 4 * NEQ;
+      T *dst = sF + threadIdx.x * 4 * NEQ;
+#pragma unroll
+      for (int v = 0; v < 4 * NEQ; ++v) dst[v] = src[v];
+      if constexpr (VISC) {
+#pragma unroll
+        for (int v = 0; v < NGS; ++v) sG[threadIdx.x * NGS + v] = gvel[c * NGS + v];
+      }
+    }
+    __syncthreads();
+  }
   if (c >= nc) return;
+  // face state of global slot `sl` of cell `cell`, from LDS when staged
+  auto slot_ptr = [&](int64_t cell, int64_t sl) -> const T * {
+    if (STAGE && cell >= c0 && cell < c0 + kBlock) return sF + (sl - c0 * 4) * NEQ;
+    return face + sl * NEQ;
+  };
+  auto grad_ptr = [&](int64_t cell) -> const T * {
+    if (STAGE && cell >= c0 && cell < c0 + kBlock) return sG + (cell - c0) * NGS;
+    return gvel + cell * NGS;
+  };
   const fk::Mat<double> md = fk::make_mat<double>(P);
   const fk::Mat<T> mt = fk::make_mat<T>(P);
   constexpr int iu = Model<NEQ>::iu;
@@ -307,10 +455,16 @@ __global__ void __launch_bounds__(kBlock, (MODE == 1 ? 2 : 3)) pass_f(fk_params 
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) rhs[v] = 0.0;
   T sig = (T)-1e9;
-  // inviscid part: HLLC, non-conservative terms, pressure correction (and the
-  // signal speed); the viscous part follows in a second loop over the faces,
-  // which keeps the register footprint of each part separate
-#pragma unroll 1
+  // viscous contributions are summed apart (rv) and added to the inviscid sums
+  // after the face loop: the canonical order shared with the oracle
+  double rv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+  T Wc1[NEQ];  // first-order state of this cell (viscous boundary ghost)
+  if constexpr (VISC) {
+    double t[NEQ];
+    fk::cons2prim<NEQ>(md, Uc, t);
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) Wc1[v] = (T)t[v];
+  }
   for (int j = 0; j < 4; ++j) {
     const int64_t s = c * 4 + j;
     const int nb = nbr[s];
@@ -320,9 +474,9 @@ __global__ void __launch_bounds__(kBlock, (MODE == 1 ? 2 : 3)) pass_f(fk_params 
     const T n[3] = {ga.x, ga.y, ga.z};
     const T A = ga.w;
     T L[NEQ], R[NEQ];
-    const T *Fs = face + s * NEQ;
+    const T *Fs = slot_ptr(c, s);
     if (nb >= 0) {
-      const T *Fn = face + (int64_t)(mv >> 1) * NEQ;
+      const T *Fn = slot_ptr(nb, (int64_t)(mv >> 1));
       if (own) {
 #pragma unroll
         for (int v = 0; v < NEQ; ++v) {
@@ -372,107 +526,80 @@ __global__ void __launch_bounds__(kBlock, (MODE == 1 ? 2 : 3)) pass_f(fk_params 
       const T cm = fmax(cl, cr);
       if (!(sig >= cm)) sig = cm;
     }
-  }
-  if constexpr (VISC) {
-    T Wc1[NEQ];  // first-order state of this cell (viscous boundary ghost)
-    {
-      double t[NEQ];
-      fk::cons2prim<NEQ>(md, Uc, t);
+    if constexpr (VISC) {
+      const T *Gc = grad_ptr(c);
+      const T *Gn = nb >= 0 ? grad_ptr(nb) : Gc;
+      const T *G0 = own ? Gc : Gn;  // owner's gradient first, as (g_o + g_n)/2
+      const T *G1 = own ? Gn : Gc;
+      T gm[3][3];
 #pragma unroll
-      for (int v = 0; v < NEQ; ++v) Wc1[v] = (T)t[v];
-    }
-#pragma unroll 1
-    for (int j = 0; j < 4; ++j) {
-      const int64_t s = c * 4 + j;
-      const int nb = nbr[s];
-      const int mv = mir[s];
-      const bool own = mv & 1;
-      const V4<T> ga = ld4<T>(geo, s);
-      const T n[3] = {ga.x, ga.y, ga.z};
-      const T A = ga.w;
-      T L[NEQ], R[NEQ];
-      const T *Fs = face + s * NEQ;
+      for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) gm[i][d] = (G0[i * 3 + d] + G1[i * 3 + d]) / 2;
+      T eddy = 0;
+      if (P.turbulence == 1) eddy = fk::wale<T>(gm, wls[s]);
+      const T ux = gm[0][0], uy = gm[0][1], uz = gm[0][2], vx = gm[1][0], vy = gm[1][1], vz = gm[1][2];
+      const T wx = gm[2][0], wy = gm[2][1], wz = gm[2][2];
+      T st[3][3];
+      st[0][0] = (2 * ux - (T)(2.0 / 3.0) * (ux + vy + wz));
+      st[1][1] = (2 * vy - (T)(2.0 / 3.0) * (ux + vy + wz));
+      st[2][2] = (2 * wz - (T)(2.0 / 3.0) * (ux + vy + wz));
+      st[0][1] = st[1][0] = (uy + vx);
+      st[0][2] = st[2][0] = (uz + wx);
+      st[1][2] = st[2][1] = (vz + wy);
+      // side-1 state: reconstructed neighbour, or the first-order ghost on boundaries
+      T R1[NEQ];
       if (nb >= 0) {
-        const T *Fn = face + (int64_t)(mv >> 1) * NEQ;
 #pragma unroll
-        for (int v = 0; v < NEQ; ++v) {
-          L[v] = own ? Fs[v] : Fn[v];
-          R[v] = own ? Fn[v] : Fs[v];
-        }
+        for (int v = 0; v < NEQ; ++v) R1[v] = R[v];
       } else {
 #pragma unroll
-        for (int v = 0; v < NEQ; ++v) L[v] = Fs[v];
+        for (int v = 0; v < NEQ; ++v) R1[v] = Wc1[v];
+        fk::ghost<NEQ, T>(mt, -nb, n, R1);
       }
-      {
-        const T *Gc = gvel + c * NG;
-        const T *Gn = nb >= 0 ? gvel + (int64_t)nb * NG : Gc;
-        const T *G0 = own ? Gc : Gn;  // owner's gradient first, as (g_o + g_n)/2
-        const T *G1 = own ? Gn : Gc;
-        T gm[3][3];
+      T mu;
+      if constexpr (NEQ == 5) {
+        const T rhoF = (L[0] + R1[0]) / 2;
+        mu = mt.mu[0] + rhoF * eddy;
+      } else {
+        const T a1 = L[0], r1 = a1 * L[1] + (1 - a1) * L[2];
+        const T a2 = R1[0], r2 = a2 * R1[1] + (1 - a2) * R1[2];
+        const T rhoF = (r1 + r2) / 2, avfF = (a1 + a2) / 2;
+        mu = avfF * mt.mu[0] + (1 - avfF) * mt.mu[1] + rhoF * eddy;
+      }
+      T fm[3];
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
+      for (int i = 0; i < 3; ++i) fm[i] = (mu * st[i][0]) * n[0] + (mu * st[i][1]) * n[1] + (mu * st[i][2]) * n[2];
+      T vf[3];
 #pragma unroll
-          for (int d = 0; d < 3; ++d) gm[i][d] = (G0[i * 3 + d] + G1[i * 3 + d]) / 2;
-        T eddy = 0;
-        if (P.turbulence == 1) eddy = fk::wale<T>(gm, wls[s]);
-        const T ux = gm[0][0], uy = gm[0][1], uz = gm[0][2], vx = gm[1][0], vy = gm[1][1], vz = gm[1][2];
-        const T wx = gm[2][0], wy = gm[2][1], wz = gm[2][2];
-        T st[3][3];
-        st[0][0] = (2 * ux - (T)(2.0 / 3.0) * (ux + vy + wz));
-        st[1][1] = (2 * vy - (T)(2.0 / 3.0) * (ux + vy + wz));
-        st[2][2] = (2 * wz - (T)(2.0 / 3.0) * (ux + vy + wz));
-        st[0][1] = st[1][0] = (uy + vx);
-        st[0][2] = st[2][0] = (uz + wx);
-        st[1][2] = st[2][1] = (vz + wy);
-        // side-1 state: reconstructed neighbour, or the first-order ghost on boundaries
-        T R1[NEQ];
-        if (nb >= 0) {
+      for (int i = 0; i < 3; ++i) vf[i] = (L[iu + i] + R1[iu + i]) / 2;
+      T fe[3];
 #pragma unroll
-          for (int v = 0; v < NEQ; ++v) R1[v] = R[v];
-        } else {
+      for (int i = 0; i < 3; ++i) {
+        T stv = 0;
 #pragma unroll
-          for (int v = 0; v < NEQ; ++v) R1[v] = Wc1[v];
-          fk::ghost<NEQ, T>(mt, -nb, n, R1);
-        }
-        T mu;
-        if constexpr (NEQ == 5) {
-          const T rhoF = (L[0] + R1[0]) / 2;
-          mu = mt.mu[0] + rhoF * eddy;
-        } else {
-          const T a1 = L[0], r1 = a1 * L[1] + (1 - a1) * L[2];
-          const T a2 = R1[0], r2 = a2 * R1[1] + (1 - a2) * R1[2];
-          const T rhoF = (r1 + r2) / 2, avfF = (a1 + a2) / 2;
-          mu = avfF * mt.mu[0] + (1 - avfF) * mt.mu[1] + rhoF * eddy;
-        }
-        T fm[3];
+        for (int d = 0; d < 3; ++d) stv += st[i][d] * vf[d];
+        fe[i] = (mu * stv) * n[i];
+      }
+      const T en = fe[0] + fe[1] + fe[2];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) fm[i] = (mu * st[i][0]) * n[0] + (mu * st[i][1]) * n[1] + (mu * st[i][2]) * n[2];
-        T vf[3];
+      for (int i = 0; i < 3; ++i) rv[i] += (double)(own ? fm[i] * A : -(fm[i]) * A);
+      rv[3] += (double)(own ? en * A : -(en) * A);
+      if constexpr (HEAT) {
+        const T k = mt.cp[0] * mu / mt.Pr0;
+        T gf[3];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) vf[i] = (L[iu + i] + R1[iu + i]) / 2;
-        T fe[3];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) {
-          T stv = 0;
-#pragma unroll
-          for (int d = 0; d < 3; ++d) stv += st[i][d] * vf[d];
-          fe[i] = (mu * stv) * n[i];
-        }
-        const T en = fe[0] + fe[1] + fe[2];
-#pragma unroll
-        for (int i = 0; i < 3; ++i) rhs[iu + i] += (double)(own ? fm[i] * A : -(fm[i]) * A);
-        constexpr int ie = NEQ == 5 ? 4 : 8;
-        rhs[ie] += (double)(own ? en * A : -(en) * A);
-        if constexpr (HEAT) {
-          const T k = mt.cp[0] * mu / mt.Pr0;
-          T gf[3];
-#pragma unroll
-          for (int d = 0; d < 3; ++d) gf[d] = (G0[9 + d] + G1[9 + d]) / 2;
-          const T q = k * (gf[0] * n[0] + gf[1] * n[1] + gf[2] * n[2]);
-          rhs[4] += (double)(own ? q * A : -(q) * A);
-        }
+        for (int d = 0; d < 3; ++d) gf[d] = (G0[9 + d] + G1[9 + d]) / 2;
+        const T q = k * (gf[0] * n[0] + gf[1] * n[1] + gf[2] * n[2]);
+        rv[4] += (double)(own ? q * A : -(q) * A);
       }
     }
+  }
+  if constexpr (VISC) {
+#pragma unroll
+    for (int i = 0; i < 3; ++i) rhs[iu + i] += rv[i];
+    rhs[NEQ == 5 ? 4 : 8] += rv[3];
+    if constexpr (HEAT) rhs[4] += rv[4];
   }
   if constexpr (MODE != 0) {
 #pragma unroll
@@ -571,7 +698,8 @@ using fk::chem_dev;
 
 __global__ void __launch_bounds__(kBlock) relax_therm(fk_params P, int64_t nc, double *__restrict__ U,
                                                       uint32_t *__restrict__ list,
-                                                      unsigned long long *__restrict__ counts) {
+                                                      unsigned long long *__restrict__ counts,
+                                                      unsigned long long *__restrict__ part) {
   const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int l = threadIdx.x & 63;
   int chem = 0, it = 0, ts = 0;
@@ -604,16 +732,8 @@ __global__ void __launch_bounds__(kBlock) relax_therm(fk_params P, int64_t nc, d
     base = __shfl(base, leader, 64);
     if (chem) list[base + __popcll(mask & ((1ull << l) - 1ull))] = (uint32_t)c;
   }
-  unsigned long long b = it, d = ts;
-  for (int off = 32; off > 0; off >>= 1) {
-    b += __shfl_down(b, off, 64);
-    d += __shfl_down(d, off, 64);
-  }
-  if (l == 0) {
-    if (npush) atomicAdd(&counts[0], (unsigned long long)npush);
-    atomicAdd(&counts[1], b);
-    atomicAdd(&counts[2], d);
-  }
+  unsigned long long v[3] = {(unsigned long long)chem, (unsigned long long)it, (unsigned long long)ts};
+  fk::block_partials<3>(v, part);  // [n_chem, newton iterations, tsat iterations]
 }
 
 __global__ void __launch_bounds__(kBlock) relax_chem(fk_params P, double *__restrict__ U,
@@ -721,13 +841,14 @@ int upload_geometry(fk_ctx *c) {
 void free_all(fk_ctx *c) {
   void *ptrs[] = {c->nbr, c->mir, c->geo, c->rvec, c->wls, c->vol, c->volcfl, c->U[0], c->U[1], c->U[2], c->face,
                   c->gvel, c->rhs, c->dtc, c->red, c->dscal, c->dgrad, c->dtheta, c->lcount, c->scode, c->hoff,
-                  c->hcell, c->rlist};
+                  c->hcell, c->rlist, c->rpart};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->nbr = c->mir = nullptr;
   c->scode = c->hoff = c->hcell = nullptr;
   c->halo_tab = 0;
   c->rlist = nullptr;
+  c->rpart = nullptr;
   c->rlist_cap = 0;
   c->geo = c->rvec = c->wls = nullptr;
   c->vol = c->volcfl = nullptr;
@@ -749,10 +870,17 @@ int launch_pass_g_t(fk_ctx *c) {
   T *gv = (T *)c->gvel;
   const int S = kBlock + c->halo_tab;
   const size_t lds = (size_t)S * c->neq * sizeof(T);
-#define PG(NEQ, VISC, HEAT)                                                                                          \
-  LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g<NEQ, T, VISC, HEAT>), dim3(gr), dim3(kBlock), lds, c->stream,  \
-                                               c->p, c->nc, U, c->scode, c->hoff, c->hcell, S, c->geo, c->rvec,      \
-                                               c->vol, face, gv, dg, dth))
+#define PG(NEQ, VISC, HEAT)                                                                                            \
+  do {                                                                                                                 \
+    if (c->stage_g)                                                                                                    \
+      LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g<NEQ, T, VISC, HEAT>), dim3(gr), dim3(kBlock), lds,           \
+                                                   c->stream, c->p, c->nc, U, c->scode, c->hoff, c->hcell, S, c->geo,   \
+                                                   c->rvec, c->vol, face, gv, dg, dth));                                \
+    else                                                                                                               \
+      LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g_direct<NEQ, T, VISC, HEAT>), dim3(gr), dim3(kBlock), 0,      \
+                                                   c->stream, c->p, c->nc, U, c->nbr, c->mir, c->geo, c->rvec, c->vol,  \
+                                                   face, gv, dg, dth));                                                 \
+  } while (0)
   if (c->neq == 5) {
     if (heat) PG(5, true, true);
     else if (visc) PG(5, true, false);
@@ -775,10 +903,20 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
   const T *gv = (const T *)c->gvel;
   const T *wl = (const T *)c->wls;
   const char *nm = MODE == 0 ? "pass_f" : (MODE == 1 ? "pass_f_rhs" : "pass_f_res");
-#define PF(NEQ, VISC, HEAT)                                                                                       \
-  LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE>), dim3(gr), dim3(kBlock), 0, c->stream, \
-                                         c->p, c->nc, U, U0, Unew, c->nbr, c->mir, c->geo, wl, c->vol, c->volcfl,  \
-                                         face, gv, c->rhs, c->dtc, irk, dt, dtp))
+  // LDS staging of face states (mixed precision only: 40 KB + 9 KB per workgroup)
+  const bool stage = c->stage_f && sizeof(T) == 4;
+  const size_t lds = stage ? (size_t)kBlock * (4 * c->neq + 12) * sizeof(T) : 0;
+#define PF(NEQ, VISC, HEAT)                                                                                            \
+  do {                                                                                                                 \
+    if (stage)                                                                                                         \
+      LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, true>), dim3(gr), dim3(kBlock), lds,     \
+                                             c->stream, c->p, c->nc, U, U0, Unew, c->nbr, c->mir, c->geo, wl, c->vol,   \
+                                             c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp));                       \
+    else                                                                                                               \
+      LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false>), dim3(gr), dim3(kBlock), 0,      \
+                                             c->stream, c->p, c->nc, U, U0, Unew, c->nbr, c->mir, c->geo, wl, c->vol,   \
+                                             c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp));                       \
+  } while (0)
   if (c->neq == 5) {
     if (heat) PF(5, true, true);
     else if (visc) PF(5, true, false);
@@ -868,9 +1006,12 @@ int ensure_rlist(fk_ctx *c) {
   if (c->nc >= (int64_t(1) << 31)) return fail(FK_ERANGE, "relaxation supports < 2^31 cells");
   if (c->rlist_cap >= c->nc) return FK_OK;
   if (c->rlist) (void)hipFree(c->rlist);
+  if (c->rpart) (void)hipFree(c->rpart);
   c->rlist = nullptr;
+  c->rpart = nullptr;
   c->rlist_cap = 0;
   HIPCHK(hipMalloc(&c->rlist, (size_t)c->nc * sizeof(uint32_t)));
+  HIPCHK(hipMalloc(&c->rpart, (size_t)grid_for(c->nc) * 4 * sizeof(unsigned long long)));
   c->rlist_cap = c->nc;
   return FK_OK;
 }
@@ -893,7 +1034,9 @@ int do_relax(fk_ctx *c, fk_relax_stats *st, int mode = FK_RELAX_PLAIN) {
     if (rc) return rc;
     LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_ull, dim3(1), dim3(64), 0, c->stream, cnt, 6));
     LAUNCH_TIMED(c, "relax", hipLaunchKernelGGL(relax_therm, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, c->p,
-                                                c->nc, c->U[c->ia], c->rlist, cnt));
+                                                c->nc, c->U[c->ia], c->rlist, cnt, c->rpart));
+    LAUNCH_TIMED(c, "relax_count", hipLaunchKernelGGL(fk::reduce_partials<3>, dim3(1), dim3(256), 0, c->stream,
+                                                      c->rpart, (int64_t)grid_for(c->nc), cnt));
     const unsigned gch = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_for(c->nc), 1024));
     LAUNCH_TIMED(c, "relax_chem", hipLaunchKernelGGL(relax_chem, dim3(gch), dim3(kBlock), 0, c->stream, c->p,
                                                      c->U[c->ia], c->rlist, cnt));
@@ -1217,6 +1360,14 @@ int fk_set_tuning(fk_ctx *c, const char *name, int64_t value) {
   if (!c || !name) return fail(FK_EINVAL, "null argument");
   if (std::strcmp(name, "fuse_update") == 0) {
     c->fuse_update = value != 0;
+    return FK_OK;
+  }
+  if (std::strcmp(name, "stage_f") == 0) {
+    c->stage_f = value != 0;
+    return FK_OK;
+  }
+  if (std::strcmp(name, "stage_g") == 0) {
+    c->stage_g = value != 0;
     return FK_OK;
   }
   return fail(FK_EINVAL, std::string("unknown tuning option ") + name);

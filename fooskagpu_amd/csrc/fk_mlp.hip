@@ -385,7 +385,8 @@ constexpr int kColdIters = 4;
 
 __global__ void __launch_bounds__(256) relax_screen_k(fk_params P, int64_t nc, double *__restrict__ U,
                                                       uint32_t *__restrict__ list,
-                                                      unsigned long long *__restrict__ counts) {
+                                                      unsigned long long *__restrict__ counts,
+                                                      unsigned long long *__restrict__ part) {
   const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int l = threadIdx.x & 63;
   int chem = 0, it = 0, push = 0;
@@ -430,15 +431,8 @@ __global__ void __launch_bounds__(256) relax_screen_k(fk_params P, int64_t nc, d
     const int rank = __popcll(mask & ((1ull << l) - 1ull));
     list[base + rank] = (uint32_t)((c << 1) | chem);
   }
-  unsigned long long a = chem, b = it;
-  for (int off = 32; off > 0; off >>= 1) {
-    a += __shfl_down(a, off, 64);
-    b += __shfl_down(b, off, 64);
-  }
-  if (l == 0) {
-    if (a) atomicAdd(&counts[0], a);
-    atomicAdd(&counts[1], b);
-  }
+  unsigned long long v[2] = {(unsigned long long)chem, (unsigned long long)it};
+  fk::block_partials<2>(v, part);  // [n_chem, newton iterations]
 }
 
 // Surrogate relaxation, stage 2 (the work list, 64 entries per wave): MLP
@@ -637,8 +631,11 @@ int fk_relax_mlp_impl(fk_ctx *c, bool warm) {
   if (rc) return rc;
   unsigned long long *cnt = (unsigned long long *)c->lcount;
   LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_u64_k, dim3(1), dim3(64), 0, c->stream, cnt, 6));
-  LAUNCH_TIMED(c, "relax_screen", hipLaunchKernelGGL(relax_screen_k, dim3((unsigned)((c->nc + 255) / 256)), dim3(256),
-                                                     0, c->stream, c->p, c->nc, c->U[c->ia], c->rlist, cnt));
+  const unsigned gs = (unsigned)((c->nc + 255) / 256);
+  LAUNCH_TIMED(c, "relax_screen", hipLaunchKernelGGL(relax_screen_k, dim3(gs), dim3(256), 0, c->stream, c->p, c->nc,
+                                                     c->U[c->ia], c->rlist, cnt, c->rpart));
+  LAUNCH_TIMED(c, "relax_count", hipLaunchKernelGGL(fk::reduce_partials<2>, dim3(1), dim3(256), 0, c->stream, c->rpart,
+                                                    (int64_t)gs, cnt));
   const int64_t nchunk = (c->nc + 63) / 64;
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nchunk + 3) / 4, 1024));
   if (warm)

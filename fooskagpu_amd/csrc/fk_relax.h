@@ -545,4 +545,46 @@ FK_HD inline bool mlp_targets(const fk_params &P, const double *U, float *y) {
   return ok;
 }
 
+// ---------------- relaxation counters without same-address atomics ----------------
+// Every wave adding its counts to one global counter serialises ~10^6
+// atomics on one L2 channel (several ms at 48M cells).  Instead each
+// 256-thread workgroup writes its K sums to part[block][K] and one small
+// kernel adds the column sums to the counters.
+
+template <int K>
+__device__ inline void block_partials(unsigned long long (&v)[K], unsigned long long *__restrict__ part) {
+  __shared__ unsigned long long sh[4][K];
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[w][k] = v[k];
+  __syncthreads();
+  if ((int)threadIdx.x < K) {
+    const int k = threadIdx.x;
+    part[(int64_t)blockIdx.x * K + k] = sh[0][k] + sh[1][k] + sh[2][k] + sh[3][k];
+  }
+}
+
+// counts[k] += sum over blocks of part[b][k], k < K (one 256-thread block)
+template <int K>
+__global__ void __launch_bounds__(256) reduce_partials(const unsigned long long *__restrict__ part, int64_t nb,
+                                                       unsigned long long *__restrict__ counts) {
+  __shared__ unsigned long long sh[256];
+  for (int k = 0; k < K; ++k) {
+    unsigned long long s = 0;
+    for (int64_t b = threadIdx.x; b < nb; b += 256) s += part[b * K + k];
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+      if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) counts[k] += sh[0];
+    __syncthreads();
+  }
+}
+
 }  // namespace fk

@@ -149,7 +149,11 @@ int fk_set_debug(fk_ctx *ctx, int on);
  * counterpart).  "fuse_update" (default 0): 0 = pass F stores the residual
  * and a separate streaming kernel applies the RK stage update (faster on
  * MI355X: 8.95 + 2.44 ms vs 13.7 ms at 48M cells, mixed); 1 = the update runs
- * inside pass F. */
+ * inside pass F.  "stage_f" (default 1): mixed precision only; pass F copies
+ * the face states and velocity gradients of its workgroup's cells to LDS and
+ * reads in-workgroup mirror states from there.  "stage_g" (default 0): pass G
+ * converts each workgroup's cells and halo cells to primitive variables once
+ * into an LDS table (else every cell converts its 4 neighbours itself). */
 int fk_set_tuning(fk_ctx *ctx, const char *name, int64_t value);
 
 /* ---- surrogate of the relaxation Newton solve (no counterpart in the

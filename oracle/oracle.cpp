@@ -1092,19 +1092,22 @@ int orc_compute_rhs(orc *o) {
       if (!(sig >= cMax)) sig = cMax;  // atomicMaxDouble
     }
     o->signal[(size_t)c] = sig;
-    // viscous contributions after the inviscid ones, again in ascending face
-    // order (the reference adds both with atomics, in no fixed order)
+    // viscous contributions summed separately in ascending face order, then
+    // added to the inviscid sums (the reference adds both with atomics, in
+    // no fixed order)
     if (visc) {
+      double rv[5] = {0, 0, 0, 0, 0};  // momentum x,y,z, energy, heat
       for (int k = 0; k < 4; ++k) {
         int64_t f = o->cf[(size_t)(4 * c + k)];
         int side = o->fc2el[(size_t)(2 * f)] == c ? 0 : 1;
         double A = o->area[(size_t)f];
         const double *vv = &o->visc[(size_t)(f * 5)];
-        for (int i = 0; i < 3; ++i) r[iu + i] += side == 0 ? vv[i] * A : -(vv[i]) * A;
-        int ie = single ? 4 : 8;
-        r[ie] += side == 0 ? vv[3] * A : -(vv[3]) * A;
-        if (single && p.heatFlux == 1) r[4] += side == 0 ? vv[4] * A : -(vv[4]) * A;
+        for (int i = 0; i < 4; ++i) rv[i] += side == 0 ? vv[i] * A : -(vv[i]) * A;
+        if (single && p.heatFlux == 1) rv[4] += side == 0 ? vv[4] * A : -(vv[4]) * A;
       }
+      for (int i = 0; i < 3; ++i) r[iu + i] += rv[i];
+      r[single ? 4 : 8] += rv[3];
+      if (single && p.heatFlux == 1) r[4] += rv[4];
     }
   }
   return FK_OK;
