@@ -116,6 +116,8 @@ HIP_API = [
     ("fk_cons_calc", ctypes.c_int, [_ctx, c_double_p]),
     ("fk_run_steps", ctypes.c_int, [_ctx, ctypes.c_int, c_double_p, c_double_p]),
     ("fk_sync", ctypes.c_int, [_ctx]),
+    ("fk_get_clock", ctypes.c_int, [_ctx, c_double_p, c_double_p, ctypes.POINTER(ctypes.c_int64)]),
+    ("fk_set_clock", ctypes.c_int, [_ctx, ctypes.c_double, ctypes.c_double, ctypes.c_int64]),
     ("fk_download", ctypes.c_int, [_ctx, ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int64]),
     ("fk_set_debug", ctypes.c_int, [_ctx, ctypes.c_int]),
     ("fk_set_tuning", ctypes.c_int, [_ctx, ctypes.c_char_p, ctypes.c_int64]),

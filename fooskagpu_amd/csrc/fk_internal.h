@@ -75,6 +75,11 @@ struct fk_ctx {
   double *U[3] = {nullptr, nullptr, nullptr};
   int ia = 0, iu0 = 0;
   void *face = nullptr, *gvel = nullptr;
+  // primitive states [C][NEQ] (T) and FP64 cell pressure [C] of U[ia], written
+  // by the kernel that produced U[ia] (w_valid) or by prims before pass G
+  void *wprim = nullptr;
+  double *pcell = nullptr;
+  bool w_valid = false;
   double *rhs = nullptr, *dtc = nullptr, *red = nullptr, *dscal = nullptr;
   double *dgrad = nullptr, *dtheta = nullptr;
   // relaxation counters: [0, 5) of the last relaxation (n_chem, Newton

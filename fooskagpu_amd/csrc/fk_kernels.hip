@@ -268,11 +268,12 @@ __global__ void __launch_bounds__(kBlock) pass_g(fk_params P, int64_t nc, const 
   }
 }
 
-// Pass G without LDS staging: every cell converts its 4 neighbours itself
-// (5 cons2prim per cell) and keeps all variables in registers; full face
-// records are written once.  Same arithmetic and order as pass_g.
+// Pass G on the primitive states Wp written by the kernel that produced U
+// (prims / rk_update_prims): own and neighbour states are loads, no cons2prim
+// here; full face records are written once.  Same arithmetic and order as
+// pass_g (which converts U itself, staging neighbours in LDS).
 template <int NEQ, typename T, bool VISC, bool HEAT>
-__global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc, const double *__restrict__ U,
+__global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc, const T *__restrict__ Wp,
                                                  const int32_t *__restrict__ nbr, const int32_t *__restrict__ mir,
                                                  const void *__restrict__ geo, const void *__restrict__ rvec,
                                                  const double *__restrict__ vol, T *__restrict__ face,
@@ -280,13 +281,10 @@ __global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc,
                                                  double *__restrict__ dtheta) {
   const int64_t c = xcd_block() * kBlock + threadIdx.x;
   if (c >= nc) return;
-  const fk::Mat<double> md = fk::make_mat<double>(P);
   const fk::Mat<T> mt = fk::make_mat<T>(P);
-  double Wd[NEQ];
-  fk::cons2prim<NEQ>(md, U + c * NEQ, Wd);
   T W[NEQ];
 #pragma unroll
-  for (int v = 0; v < NEQ; ++v) W[v] = (T)Wd[v];
+  for (int v = 0; v < NEQ; ++v) W[v] = Wp[c * NEQ + v];
   T g[NEQ][3], mn[NEQ], mx[NEQ];
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) {
@@ -304,10 +302,8 @@ __global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc,
     const T n[3] = {ga.x, ga.y, ga.z};
     T Wn[NEQ];
     if (nb >= 0) {
-      double t[NEQ];
-      fk::cons2prim<NEQ>(md, U + (int64_t)nb * NEQ, t);
 #pragma unroll
-      for (int v = 0; v < NEQ; ++v) Wn[v] = (T)t[v];
+      for (int v = 0; v < NEQ; ++v) Wn[v] = Wp[(int64_t)nb * NEQ + v];
     } else {
 #pragma unroll
       for (int v = 0; v < NEQ; ++v) Wn[v] = W[v];
@@ -332,14 +328,12 @@ __global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc,
   T th[NEQ];
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) th[v] = (T)1e9;
-  V4<T> rr[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) rr[j] = ld4<T>(rvec, c * 4 + j);
-#pragma unroll
+#pragma unroll 1
   for (int j = 0; j < 4; ++j) {
+    const V4<T> r = ld4<T>(rvec, c * 4 + j);
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) {
-      const T wu = W[v] + rr[j].x * g[v][0] + rr[j].y * g[v][1] + rr[j].z * g[v][2];
+      const T wu = W[v] + r.x * g[v][0] + r.y * g[v][1] + r.z * g[v][2];
       T phi;
       if (wu - W[v] != 0)
         phi = (wu > W[v]) ? (mx[v] - W[v]) / (2 * (wu - W[v])) : (mn[v] - W[v]) / (2 * (wu - W[v]));
@@ -350,12 +344,13 @@ __global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc,
     }
   }
   // MUSCL face states
-#pragma unroll
+#pragma unroll 1
   for (int j = 0; j < 4; ++j) {
+    const V4<T> r = ld4<T>(rvec, c * 4 + j);
     T *out = face + (c * 4 + j) * NEQ;
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) {
-      const T ext = rr[j].x * g[v][0] + rr[j].y * g[v][1] + rr[j].z * g[v][2];
+      const T ext = r.x * g[v][0] + r.y * g[v][1] + r.z * g[v][2];
       out[v] = W[v] + th[v] * ext;
     }
   }
@@ -402,6 +397,7 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
                                                  const void *__restrict__ geo, const T *__restrict__ wls,
                                                  const double *__restrict__ vol, const double *__restrict__ volcfl,
                                                  const T *__restrict__ face, const T *__restrict__ gvel,
+                                                 const T *__restrict__ Wp, const double *__restrict__ pcell,
                                                  double *__restrict__ rhs_out, double *__restrict__ dtc, int irk,
                                                  double dt_val, const double *__restrict__ dt_ptr) {
   constexpr int NGS = ngrad<NEQ, HEAT>();
@@ -438,18 +434,16 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   constexpr int iu = Model<NEQ>::iu;
   constexpr int NG = ngrad<NEQ, HEAT>();
   const double *Uc = U + c * NEQ;
-  // side terms of this cell (fluxSource / computeCenterPress, two-phase)
+  // side terms of this cell (fluxSource / computeCenterPress, two-phase):
+  // alpha, p_L, p_G of the primitive state and the FP64 cell pressure
+  // alpha p_L + (1 - alpha) p_G, both written with U by prims / rk_update_prims
   T avf = 0, pLiq = 0, pGas = 0;
   double pCd = 0;
   if constexpr (NEQ == 10) {
-    const double a = fk::clamp_avf(md, Uc[0]);
-    const double rl = Uc[1] / a, rg = Uc[2] / (1 - a);
-    const double el = Uc[6] / (a * rl), eg = Uc[7] / ((1 - a) * rg);
-    const double pl = fk::p_of(md, 0, rl, el), pg = fk::p_of(md, 1, rg, eg);
-    pCd = a * pl + (1 - a) * pg;
-    avf = (T)a;
-    pLiq = (T)pl;
-    pGas = (T)pg;
+    pCd = pcell[c];
+    avf = Wp[c * NEQ + 0];
+    pLiq = Wp[c * NEQ + 6];
+    pGas = Wp[c * NEQ + 7];
   }
   double rhs[NEQ];
 #pragma unroll
@@ -460,10 +454,8 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   double rv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   T Wc1[NEQ];  // first-order state of this cell (viscous boundary ghost)
   if constexpr (VISC) {
-    double t[NEQ];
-    fk::cons2prim<NEQ>(md, Uc, t);
 #pragma unroll
-    for (int v = 0; v < NEQ; ++v) Wc1[v] = (T)t[v];
+    for (int v = 0; v < NEQ; ++v) Wc1[v] = Wp[c * NEQ + v];
   }
   for (int j = 0; j < 4; ++j) {
     const int64_t s = c * 4 + j;
@@ -622,6 +614,60 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
       Unew[c * NEQ + v] = u;
     }
   }
+}
+
+// Primitive states of U for passes G and F (cons2prim.h, FP64, stored as T)
+// and, two-phase, the FP64 cell pressure alpha p_L + (1 - alpha) p_G of the
+// pressure correction (computeRHS.cu:12-22).
+template <int NEQ, typename T>
+__device__ __forceinline__ void store_prims(const fk::Mat<double> &md, int64_t c, const double *u, T *__restrict__ Wp,
+                                            double *__restrict__ pcell) {
+  double s[NEQ];
+  fk::cons2prim<NEQ>(md, u, s);
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) Wp[c * NEQ + v] = (T)s[v];
+  if constexpr (NEQ == 10) pcell[c] = s[0] * s[6] + (1 - s[0]) * s[7];
+}
+
+template <int NEQ, typename T>
+__global__ void __launch_bounds__(kBlock) prims(fk_params P, int64_t nc, const double *__restrict__ U,
+                                                T *__restrict__ Wp, double *__restrict__ pcell) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= nc) return;
+  const fk::Mat<double> md = fk::make_mat<double>(P);
+  double u[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) u[v] = U[c * NEQ + v];
+  store_prims<NEQ, T>(md, c, u, Wp, pcell);
+}
+
+// RK stage (update.cu:19-103) from a stored residual, one cell per thread,
+// followed by the primitive states of the new state.
+template <int NEQ, typename T>
+__global__ void __launch_bounds__(kBlock) rk_update_prims(fk_params P, int64_t nc, const double *__restrict__ U,
+                                                          const double *__restrict__ U0, double *__restrict__ Unew,
+                                                          const double *__restrict__ rhs, const double *__restrict__ vol,
+                                                          int irk, double dt_val, const double *__restrict__ dt_ptr,
+                                                          T *__restrict__ Wp, double *__restrict__ pcell) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= nc) return;
+  const double dt = dt_ptr ? *dt_ptr : dt_val;
+  const double Vc = vol[c];
+  constexpr int iz = NEQ == 5 ? 3 : 5;
+  double un[NEQ];
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) {
+    const int64_t i = c * NEQ + v;
+    const int neglect = (v == iz && P.neglectZ == 1);
+    double u;
+    if (P.rkSteps == 1) u = (1 - neglect) * (U[i] + dt * (1.0 / Vc) * rhs[i]);
+    else if (irk == 0) u = (1 - neglect) * (U0[i] + dt * (1.0 / Vc) * rhs[i]);
+    else if (irk == 1) u = (1 - neglect) * ((3.0 / 4.0) * U0[i] + (1.0 / 4.0) * U[i] + (1.0 / 4.0) * dt * (1.0 / Vc) * rhs[i]);
+    else u = (1 - neglect) * ((1.0 / 3.0) * U0[i] + (2.0 / 3.0) * U[i] + (2.0 / 3.0) * dt * (1.0 / Vc) * rhs[i]);
+    Unew[i] = u;
+    un[v] = u;
+  }
+  store_prims<NEQ, T>(fk::make_mat<double>(P), c, un, Wp, pcell);
 }
 
 // RK stage from a stored residual (update.cu:19-103).
@@ -841,7 +887,7 @@ int upload_geometry(fk_ctx *c) {
 void free_all(fk_ctx *c) {
   void *ptrs[] = {c->nbr, c->mir, c->geo, c->rvec, c->wls, c->vol, c->volcfl, c->U[0], c->U[1], c->U[2], c->face,
                   c->gvel, c->rhs, c->dtc, c->red, c->dscal, c->dgrad, c->dtheta, c->lcount, c->scode, c->hoff,
-                  c->hcell, c->rlist, c->rpart};
+                  c->hcell, c->rlist, c->rpart, c->wprim, c->pcell};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->nbr = c->mir = nullptr;
@@ -850,6 +896,9 @@ void free_all(fk_ctx *c) {
   c->rlist = nullptr;
   c->rpart = nullptr;
   c->rlist_cap = 0;
+  c->wprim = nullptr;
+  c->pcell = nullptr;
+  c->w_valid = false;
   c->geo = c->rvec = c->wls = nullptr;
   c->vol = c->volcfl = nullptr;
   c->U[0] = c->U[1] = c->U[2] = nullptr;
@@ -878,8 +927,8 @@ int launch_pass_g_t(fk_ctx *c) {
                                                    c->rvec, c->vol, face, gv, dg, dth));                                \
     else                                                                                                               \
       LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g_direct<NEQ, T, VISC, HEAT>), dim3(gr), dim3(kBlock), 0,      \
-                                                   c->stream, c->p, c->nc, U, c->nbr, c->mir, c->geo, c->rvec, c->vol,  \
-                                                   face, gv, dg, dth));                                                 \
+                                                   c->stream, c->p, c->nc, (const T *)c->wprim, c->nbr, c->mir, c->geo, \
+                                                   c->rvec, c->vol, face, gv, dg, dth));                                \
   } while (0)
   if (c->neq == 5) {
     if (heat) PG(5, true, true);
@@ -911,11 +960,13 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
     if (stage)                                                                                                         \
       LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, true>), dim3(gr), dim3(kBlock), lds,     \
                                              c->stream, c->p, c->nc, U, U0, Unew, c->nbr, c->mir, c->geo, wl, c->vol,   \
-                                             c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp));                       \
+                                             c->volcfl, face, gv, (const T *)c->wprim, c->pcell, c->rhs, c->dtc, irk,   \
+                                             dt, dtp));                                                                 \
     else                                                                                                               \
       LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false>), dim3(gr), dim3(kBlock), 0,      \
                                              c->stream, c->p, c->nc, U, U0, Unew, c->nbr, c->mir, c->geo, wl, c->vol,   \
-                                             c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp));                       \
+                                             c->volcfl, face, gv, (const T *)c->wprim, c->pcell, c->rhs, c->dtc, irk,   \
+                                             dt, dtp));                                                                 \
   } while (0)
   if (c->neq == 5) {
     if (heat) PF(5, true, true);
@@ -929,11 +980,34 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
   return FK_OK;
 }
 
+template <typename T>
+int launch_prims_t(fk_ctx *c) {
+  T *Wp = (T *)c->wprim;
+  if (c->neq == 5)
+    LAUNCH_TIMED(c, "prims", hipLaunchKernelGGL((prims<5, T>), dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream,
+                                                c->p, c->nc, c->U[c->ia], Wp, c->pcell));
+  else
+    LAUNCH_TIMED(c, "prims", hipLaunchKernelGGL((prims<10, T>), dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream,
+                                                c->p, c->nc, c->U[c->ia], Wp, c->pcell));
+  c->w_valid = true;
+  return FK_OK;
+}
+
+// primitive states of U[ia] for passes G and F, unless the kernel that wrote
+// U[ia] also wrote them
+int ensure_prims(fk_ctx *c) {
+  if (c->w_valid) return FK_OK;
+  return c->p.precision == FK_MIXED ? launch_prims_t<float>(c) : launch_prims_t<double>(c);
+}
+
 int do_pass_g(fk_ctx *c) {
+  int rc = ensure_prims(c);
+  if (rc) return rc;
   return c->p.precision == FK_MIXED ? launch_pass_g_t<float>(c) : launch_pass_g_t<double>(c);
 }
 
 int do_pass_f_rhs(fk_ctx *c) {
+  if (int rc0 = ensure_prims(c)) return rc0;
   int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 1>(c, 0, 0.0, nullptr, nullptr)
                                       : launch_pass_f_t<double, 1>(c, 0, 0.0, nullptr, nullptr);
   if (rc) return rc;
@@ -946,6 +1020,7 @@ int do_update(fk_ctx *c, int irk, double dt, const double *dtp);
 // RK stage irk from pass F: fused (MODE 0) or residual (MODE 2) + rk_update,
 // as c->fuse_update selects; both give the same bits.
 int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp) {
+  if (int rc0 = ensure_prims(c)) return rc0;
   if (!c->fuse_update) {
     int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 2>(c, 0, 0.0, nullptr, nullptr)
                                         : launch_pass_f_t<double, 2>(c, 0, 0.0, nullptr, nullptr);
@@ -958,21 +1033,36 @@ int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp) {
   int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 0>(c, irk, dt, dtp, c->U[nb])
                                       : launch_pass_f_t<double, 0>(c, irk, dt, dtp, c->U[nb]);
   if (rc) return rc;
+  c->w_valid = false;
   c->ia = nb;
   c->rhs_valid = false;
   return FK_OK;
 }
 
+template <typename T>
+int launch_rk_prims_t(fk_ctx *c, int irk, double dt, const double *dtp, int nb) {
+  T *Wp = (T *)c->wprim;
+  const unsigned gr = grid_for(c->nc);
+  if (c->neq == 5)
+    LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update_prims<5, T>), dim3(gr), dim3(kBlock), 0, c->stream, c->p,
+                                                    c->nc, c->U[c->ia], c->U[c->iu0], c->U[nb], c->rhs, c->vol, irk,
+                                                    dt, dtp, Wp, c->pcell));
+  else
+    LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update_prims<10, T>), dim3(gr), dim3(kBlock), 0, c->stream,
+                                                    c->p, c->nc, c->U[c->ia], c->U[c->iu0], c->U[nb], c->rhs, c->vol,
+                                                    irk, dt, dtp, Wp, c->pcell));
+  return FK_OK;
+}
+
+// RK stage from the stored residual; also writes the primitive states of the
+// new state for the next pass G
 int do_update(fk_ctx *c, int irk, double dt, const double *dtp) {
   const int nb = next_buffer(c);
-  const unsigned gr = grid_for(c->nc * c->neq);
-  if (c->neq == 5)
-    LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update<5>), dim3(gr), dim3(kBlock), 0, c->stream, c->p, c->nc,
-                                                    c->U[c->ia], c->U[c->iu0], c->U[nb], c->rhs, c->vol, irk, dt, dtp));
-  else
-    LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update<10>), dim3(gr), dim3(kBlock), 0, c->stream, c->p, c->nc,
-                                                    c->U[c->ia], c->U[c->iu0], c->U[nb], c->rhs, c->vol, irk, dt, dtp));
+  int rc = c->p.precision == FK_MIXED ? launch_rk_prims_t<float>(c, irk, dt, dtp, nb)
+                                      : launch_rk_prims_t<double>(c, irk, dt, dtp, nb);
+  if (rc) return rc;
   c->ia = nb;
+  c->w_valid = true;
   return FK_OK;
 }
 
@@ -1020,6 +1110,7 @@ namespace {
 
 int do_relax(fk_ctx *c, fk_relax_stats *st, int mode = FK_RELAX_PLAIN) {
   if (c->p.model != FK_MODEL_TWO_PHASE) return fail(FK_EINVAL, "relaxation needs the two-phase model");
+  c->w_valid = false;  // the relaxation changes U in place
   unsigned long long *cnt = (unsigned long long *)c->lcount;
   if (mode == FK_RELAX_MLP || mode == FK_RELAX_COLD) {
     int rc = fk_relax_mlp_impl(c, mode == FK_RELAX_MLP);
@@ -1035,8 +1126,8 @@ int do_relax(fk_ctx *c, fk_relax_stats *st, int mode = FK_RELAX_PLAIN) {
     LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_ull, dim3(1), dim3(64), 0, c->stream, cnt, 6));
     LAUNCH_TIMED(c, "relax", hipLaunchKernelGGL(relax_therm, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, c->p,
                                                 c->nc, c->U[c->ia], c->rlist, cnt, c->rpart));
-    LAUNCH_TIMED(c, "relax_count", hipLaunchKernelGGL(fk::reduce_partials<3>, dim3(1), dim3(256), 0, c->stream,
-                                                      c->rpart, (int64_t)grid_for(c->nc), cnt));
+    LAUNCH_TIMED(c, "relax_count", hipLaunchKernelGGL(fk::reduce_partials<3>, dim3(fk::reduce_grid(grid_for(c->nc))),
+                                                      dim3(256), 0, c->stream, c->rpart, (int64_t)grid_for(c->nc), cnt));
     const unsigned gch = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_for(c->nc), 1024));
     LAUNCH_TIMED(c, "relax_chem", hipLaunchKernelGGL(relax_chem, dim3(gch), dim3(kBlock), 0, c->stream, c->p,
                                                      c->U[c->ia], c->rlist, cnt));
@@ -1123,6 +1214,7 @@ int fk_set_params(fk_ctx *c, const fk_params *p) {
     if (rc) return rc;
   }
   c->rhs_valid = false;
+  c->w_valid = false;
   return FK_OK;
 }
 
@@ -1241,6 +1333,8 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
   for (int i = 0; i < 3; ++i) HIPCHK(hipMalloc(&c->U[i], cs * neq * 8));
   HIPCHK(hipMalloc(&c->face, (size_t)ns * neq * 8));
   HIPCHK(hipMalloc(&c->gvel, cs * 12 * 8));
+  HIPCHK(hipMalloc(&c->wprim, cs * neq * 8));
+  HIPCHK(hipMalloc(&c->pcell, cs * 8));
   HIPCHK(hipMalloc(&c->rhs, cs * neq * 8));
   HIPCHK(hipMalloc(&c->dtc, cs * 8));
   HIPCHK(hipMalloc(&c->red, 4096 * 8));
@@ -1277,6 +1371,7 @@ int fk_upload_state(fk_ctx *c, const double *U) {
   HIPCHK(hipMemcpy(c->U[0], U, (size_t)c->nc * c->neq * 8, hipMemcpyHostToDevice));
   c->have_state = true;
   c->rhs_valid = false;
+  c->w_valid = false;
   return FK_OK;
 }
 
@@ -1454,6 +1549,23 @@ int fk_run_steps(fk_ctx *c, int nsteps, double *dt_last, double *phys_time) {
   }
   if (dt_last) *dt_last = c->dt;
   if (phys_time) *phys_time = c->time;
+  return FK_OK;
+}
+
+int fk_get_clock(fk_ctx *c, double *dt, double *time, int64_t *step) {
+  if (!c) return fail(FK_EINVAL, "null context");
+  if (dt) *dt = c->dt;
+  if (time) *time = c->time;
+  if (step) *step = c->step;
+  return FK_OK;
+}
+
+int fk_set_clock(fk_ctx *c, double dt, double time, int64_t step) {
+  if (!c) return fail(FK_EINVAL, "null context");
+  if (!(dt >= 0) || step < 0) return fail(FK_EINVAL, "dt must be >= 0 and step >= 0");
+  c->dt = dt;
+  c->time = time;
+  c->step = step;
   return FK_OK;
 }
 

@@ -634,8 +634,8 @@ int fk_relax_mlp_impl(fk_ctx *c, bool warm) {
   const unsigned gs = (unsigned)((c->nc + 255) / 256);
   LAUNCH_TIMED(c, "relax_screen", hipLaunchKernelGGL(relax_screen_k, dim3(gs), dim3(256), 0, c->stream, c->p, c->nc,
                                                      c->U[c->ia], c->rlist, cnt, c->rpart));
-  LAUNCH_TIMED(c, "relax_count", hipLaunchKernelGGL(fk::reduce_partials<2>, dim3(1), dim3(256), 0, c->stream, c->rpart,
-                                                    (int64_t)gs, cnt));
+  LAUNCH_TIMED(c, "relax_count", hipLaunchKernelGGL(fk::reduce_partials<2>, dim3(fk::reduce_grid(gs)), dim3(256), 0,
+                                                    c->stream, c->rpart, (int64_t)gs, cnt));
   const int64_t nchunk = (c->nc + 63) / 64;
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nchunk + 3) / 4, 1024));
   if (warm)

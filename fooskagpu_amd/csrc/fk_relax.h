@@ -568,23 +568,39 @@ __device__ inline void block_partials(unsigned long long (&v)[K], unsigned long 
   }
 }
 
-// counts[k] += sum over blocks of part[b][k], k < K (one 256-thread block)
+// counts[k] += sum over blocks of part[b][k], k < K.  Each 256-thread block
+// sums a contiguous chunk of the partials and adds its K totals with one
+// atomic each (a few hundred atomics in all; integer sums, so the result
+// does not depend on their order).
 template <int K>
 __global__ void __launch_bounds__(256) reduce_partials(const unsigned long long *__restrict__ part, int64_t nb,
                                                        unsigned long long *__restrict__ counts) {
-  __shared__ unsigned long long sh[256];
-  for (int k = 0; k < K; ++k) {
-    unsigned long long s = 0;
-    for (int64_t b = threadIdx.x; b < nb; b += 256) s += part[b * K + k];
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-      if ((int)threadIdx.x < st) sh[threadIdx.x] += sh[threadIdx.x + st];
-      __syncthreads();
-    }
-    if (threadIdx.x == 0) counts[k] += sh[0];
-    __syncthreads();
+  __shared__ unsigned long long sh[4][K];
+  const int64_t per = (nb + gridDim.x - 1) / gridDim.x;
+  const int64_t b0 = (int64_t)blockIdx.x * per, b1 = b0 + per < nb ? b0 + per : nb;
+  unsigned long long v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = 0;
+  for (int64_t b = b0 + threadIdx.x; b < b1; b += 256)
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] += part[b * K + k];
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    for (int off = 32; off > 0; off >>= 1) v[k] += __shfl_down(v[k], off, 64);
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int k = 0; k < K; ++k) sh[threadIdx.x >> 6][k] = v[k];
+  __syncthreads();
+  if ((int)threadIdx.x < K) {
+    const unsigned long long t = sh[0][threadIdx.x] + sh[1][threadIdx.x] + sh[2][threadIdx.x] + sh[3][threadIdx.x];
+    if (t) atomicAdd(&counts[threadIdx.x], t);
   }
+}
+
+// grid of reduce_partials for nb partials
+inline unsigned reduce_grid(int64_t nb) {
+  const int64_t g = (nb + 1023) / 1024;
+  return (unsigned)(g < 1 ? 1 : (g > 512 ? 512 : g));
 }
 
 }  // namespace fk

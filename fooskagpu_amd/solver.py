@@ -114,6 +114,15 @@ class GpuSolver:
         _lib.check_hip(_lib.hip_lib().fk_run_steps(self._h, nsteps, ctypes.byref(dt), ctypes.byref(t)))
         return dt.value, t.value
 
+    def clock(self):
+        """(dt, physical time, step counter) of the loop."""
+        dt, t, st = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+        _lib.check_hip(_lib.hip_lib().fk_get_clock(self._h, ctypes.byref(dt), ctypes.byref(t), ctypes.byref(st)))
+        return dt.value, t.value, st.value
+
+    def set_clock(self, dt, time, step):
+        _lib.check_hip(_lib.hip_lib().fk_set_clock(self._h, dt, time, step))
+
     def sync(self):
         _lib.check_hip(_lib.hip_lib().fk_sync(self._h))
 

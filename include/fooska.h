@@ -134,6 +134,13 @@ int fk_run_steps(fk_ctx *ctx, int nsteps, double *dt_last, double *phys_time);
 
 /* Block until the context's stream is idle. */
 int fk_sync(fk_ctx *ctx);
+/* Time-step state of the loop: last dt, physical time and step counter (the
+ * step counter decides when fk_run_steps recomputes dt, every adaptDtItr
+ * steps).  With fk_download_state / fk_upload_state they make a bit-exact
+ * restart point; the reference's restart (restart.cu:113-152, 424-497)
+ * writes physicalTime and consVec as %.10e text. */
+int fk_get_clock(fk_ctx *ctx, double *dt, double *time, int64_t *step);
+int fk_set_clock(fk_ctx *ctx, double dt, double time, int64_t step);
 
 /* Download an intermediate array for phase-level parity checks.  Names and
  * layouts (FP64 path; the mixed path stores FP32 where marked f):
