@@ -670,28 +670,6 @@ __global__ void __launch_bounds__(kBlock) rk_update_prims(fk_params P, int64_t n
   store_prims<NEQ, T>(fk::make_mat<double>(P), c, un, Wp, pcell);
 }
 
-// RK stage from a stored residual (update.cu:19-103).
-template <int NEQ>
-__global__ void __launch_bounds__(kBlock) rk_update(fk_params P, int64_t nc, const double *__restrict__ U,
-                                                    const double *__restrict__ U0, double *__restrict__ Unew,
-                                                    const double *__restrict__ rhs, const double *__restrict__ vol,
-                                                    int irk, double dt_val, const double *__restrict__ dt_ptr) {
-  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (i >= nc * NEQ) return;
-  const int64_t c = i / NEQ;
-  const int v = (int)(i - c * NEQ);
-  const double dt = dt_ptr ? *dt_ptr : dt_val;
-  const double Vc = vol[c];
-  constexpr int iz = NEQ == 5 ? 3 : 5;
-  const int neglect = (v == iz && P.neglectZ == 1);
-  double u;
-  if (P.rkSteps == 1) u = (1 - neglect) * (U[i] + dt * (1.0 / Vc) * rhs[i]);
-  else if (irk == 0) u = (1 - neglect) * (U0[i] + dt * (1.0 / Vc) * rhs[i]);
-  else if (irk == 1) u = (1 - neglect) * ((3.0 / 4.0) * U0[i] + (1.0 / 4.0) * U[i] + (1.0 / 4.0) * dt * (1.0 / Vc) * rhs[i]);
-  else u = (1 - neglect) * ((1.0 / 3.0) * U0[i] + (2.0 / 3.0) * U[i] + (2.0 / 3.0) * dt * (1.0 / Vc) * rhs[i]);
-  Unew[i] = u;
-}
-
 // min over cells of Vcfl/signal: per-block minima, then one block folds them
 // and forms dt = min(cfl * min, maxDt) on the device.
 __global__ void __launch_bounds__(kBlock) min_blocks(int64_t n, const double *__restrict__ x, double *__restrict__ out) {
@@ -1339,8 +1317,6 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
   HIPCHK(hipMalloc(&c->dtc, cs * 8));
   HIPCHK(hipMalloc(&c->red, 4096 * 8));
   HIPCHK(hipMalloc(&c->dscal, 16 * 8));
-  HIPCHK(hipMalloc(&c->dgrad, cs * neq * 3 * 8));
-  HIPCHK(hipMalloc(&c->dtheta, cs * neq * 8));
   HIPCHK(hipMalloc(&c->lcount, 16 * 8));
   HIPCHK(hipMemset(c->lcount, 0, 16 * 8));
   HIPCHK(hipMalloc(&c->scode, (size_t)ns * 4));
@@ -1578,6 +1554,12 @@ int fk_sync(fk_ctx *c) {
 
 int fk_set_debug(fk_ctx *c, int on) {
   if (!c) return fail(FK_EINVAL, "null context");
+  if (on && !c->have_mesh) return fail(FK_ESTATE, "upload the mesh before enabling debug output");
+  if (on && !c->dgrad) {  // gradient / limiter copies, allocated only when asked for
+    HIPCHK(hipSetDevice(c->device));
+    HIPCHK(hipMalloc(&c->dgrad, (size_t)c->nc * c->neq * 3 * 8));
+    HIPCHK(hipMalloc(&c->dtheta, (size_t)c->nc * c->neq * 8));
+  }
   c->debug = on != 0;
   return FK_OK;
 }
@@ -1596,6 +1578,7 @@ int fk_download(fk_ctx *c, const char *name, void *host, int64_t bytes) {
     HIPCHK(hipMemcpy(host, src, (size_t)nb, hipMemcpyDeviceToHost));
     return FK_OK;
   };
+  if ((s == "grad" || s == "theta") && !c->dgrad) return fail(FK_ESTATE, "enable fk_set_debug before pass G");
   if (s == "grad") return get(c->dgrad, nc * neq * 3 * 8);
   if (s == "theta") return get(c->dtheta, nc * neq * 8);
   if (s == "face") return get(c->face, nc * 4 * neq * (int64_t)ts);
