@@ -75,11 +75,6 @@ struct fk_ctx {
   double *U[3] = {nullptr, nullptr, nullptr};
   int ia = 0, iu0 = 0;
   void *face = nullptr, *gvel = nullptr;
-  // primitive states [C][NEQ] (T) and FP64 cell pressure [C] of U[ia], written
-  // by the kernel that produced U[ia] (w_valid) or by prims before pass G
-  void *wprim = nullptr;
-  double *pcell = nullptr;
-  bool w_valid = false;
   double *rhs = nullptr, *dtc = nullptr, *red = nullptr, *dscal = nullptr;
   double *dgrad = nullptr, *dtheta = nullptr;
   // relaxation counters: [0, 5) of the last relaxation (n_chem, Newton
@@ -98,6 +93,7 @@ struct fk_ctx {
   bool fuse_update = false;  // RK update inside pass F (else residual + rk_update)
   bool stage_f = true;       // pass F stages the workgroup's face states in LDS (mixed only)
   bool stage_g = false;      // pass G stages neighbour primitive states in LDS
+  bool roll_g = false;       // pass G limiter / output face loops not unrolled
   fkx::Timer tm;
   fkx::Mlp mlp;
   // host geometry kept to rebuild the precision-dependent arrays

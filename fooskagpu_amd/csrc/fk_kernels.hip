@@ -268,12 +268,11 @@ __global__ void __launch_bounds__(kBlock) pass_g(fk_params P, int64_t nc, const 
   }
 }
 
-// Pass G on the primitive states Wp written by the kernel that produced U
-// (prims / rk_update_prims): own and neighbour states are loads, no cons2prim
-// here; full face records are written once.  Same arithmetic and order as
-// pass_g (which converts U itself, staging neighbours in LDS).
-template <int NEQ, typename T, bool VISC, bool HEAT>
-__global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc, const T *__restrict__ Wp,
+// Pass G without LDS staging: every cell converts its 4 neighbours itself
+// (5 cons2prim per cell) and keeps all variables in registers; full face
+// records are written once.  Same arithmetic and order as pass_g.
+template <int NEQ, typename T, bool VISC, bool HEAT, int UNR>
+__global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc, const double *__restrict__ U,
                                                  const int32_t *__restrict__ nbr, const int32_t *__restrict__ mir,
                                                  const void *__restrict__ geo, const void *__restrict__ rvec,
                                                  const double *__restrict__ vol, T *__restrict__ face,
@@ -281,10 +280,13 @@ __global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc,
                                                  double *__restrict__ dtheta) {
   const int64_t c = xcd_block() * kBlock + threadIdx.x;
   if (c >= nc) return;
+  const fk::Mat<double> md = fk::make_mat<double>(P);
   const fk::Mat<T> mt = fk::make_mat<T>(P);
+  double Wd[NEQ];
+  fk::cons2prim<NEQ>(md, U + c * NEQ, Wd);
   T W[NEQ];
 #pragma unroll
-  for (int v = 0; v < NEQ; ++v) W[v] = Wp[c * NEQ + v];
+  for (int v = 0; v < NEQ; ++v) W[v] = (T)Wd[v];
   T g[NEQ][3], mn[NEQ], mx[NEQ];
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) {
@@ -302,8 +304,10 @@ __global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc,
     const T n[3] = {ga.x, ga.y, ga.z};
     T Wn[NEQ];
     if (nb >= 0) {
+      double t[NEQ];
+      fk::cons2prim<NEQ>(md, U + (int64_t)nb * NEQ, t);
 #pragma unroll
-      for (int v = 0; v < NEQ; ++v) Wn[v] = Wp[(int64_t)nb * NEQ + v];
+      for (int v = 0; v < NEQ; ++v) Wn[v] = (T)t[v];
     } else {
 #pragma unroll
       for (int v = 0; v < NEQ; ++v) Wn[v] = W[v];
@@ -324,11 +328,12 @@ __global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc,
       }
     }
   }
-  // limiter: theta = min over faces of clamp(phi, 0, 1)
+  // limiter: theta = min over faces of clamp(phi, 0, 1); UNR = unroll factor
+  // of the face loops below (1: lower register use, 4: more overlap)
   T th[NEQ];
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) th[v] = (T)1e9;
-#pragma unroll 1
+#pragma unroll UNR
   for (int j = 0; j < 4; ++j) {
     const V4<T> r = ld4<T>(rvec, c * 4 + j);
 #pragma unroll
@@ -344,7 +349,7 @@ __global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc,
     }
   }
   // MUSCL face states
-#pragma unroll 1
+#pragma unroll UNR
   for (int j = 0; j < 4; ++j) {
     const V4<T> r = ld4<T>(rvec, c * 4 + j);
     T *out = face + (c * 4 + j) * NEQ;
@@ -397,7 +402,6 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
                                                  const void *__restrict__ geo, const T *__restrict__ wls,
                                                  const double *__restrict__ vol, const double *__restrict__ volcfl,
                                                  const T *__restrict__ face, const T *__restrict__ gvel,
-                                                 const T *__restrict__ Wp, const double *__restrict__ pcell,
                                                  double *__restrict__ rhs_out, double *__restrict__ dtc, int irk,
                                                  double dt_val, const double *__restrict__ dt_ptr) {
   constexpr int NGS = ngrad<NEQ, HEAT>();
@@ -434,16 +438,18 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   constexpr int iu = Model<NEQ>::iu;
   constexpr int NG = ngrad<NEQ, HEAT>();
   const double *Uc = U + c * NEQ;
-  // side terms of this cell (fluxSource / computeCenterPress, two-phase):
-  // alpha, p_L, p_G of the primitive state and the FP64 cell pressure
-  // alpha p_L + (1 - alpha) p_G, both written with U by prims / rk_update_prims
+  // side terms of this cell (fluxSource / computeCenterPress, two-phase)
   T avf = 0, pLiq = 0, pGas = 0;
   double pCd = 0;
   if constexpr (NEQ == 10) {
-    pCd = pcell[c];
-    avf = Wp[c * NEQ + 0];
-    pLiq = Wp[c * NEQ + 6];
-    pGas = Wp[c * NEQ + 7];
+    const double a = fk::clamp_avf(md, Uc[0]);
+    const double rl = Uc[1] / a, rg = Uc[2] / (1 - a);
+    const double el = Uc[6] / (a * rl), eg = Uc[7] / ((1 - a) * rg);
+    const double pl = fk::p_of(md, 0, rl, el), pg = fk::p_of(md, 1, rg, eg);
+    pCd = a * pl + (1 - a) * pg;
+    avf = (T)a;
+    pLiq = (T)pl;
+    pGas = (T)pg;
   }
   double rhs[NEQ];
 #pragma unroll
@@ -454,8 +460,10 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   double rv[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
   T Wc1[NEQ];  // first-order state of this cell (viscous boundary ghost)
   if constexpr (VISC) {
+    double t[NEQ];
+    fk::cons2prim<NEQ>(md, Uc, t);
 #pragma unroll
-    for (int v = 0; v < NEQ; ++v) Wc1[v] = Wp[c * NEQ + v];
+    for (int v = 0; v < NEQ; ++v) Wc1[v] = (T)t[v];
   }
   for (int j = 0; j < 4; ++j) {
     const int64_t s = c * 4 + j;
@@ -616,58 +624,26 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   }
 }
 
-// Primitive states of U for passes G and F (cons2prim.h, FP64, stored as T)
-// and, two-phase, the FP64 cell pressure alpha p_L + (1 - alpha) p_G of the
-// pressure correction (computeRHS.cu:12-22).
-template <int NEQ, typename T>
-__device__ __forceinline__ void store_prims(const fk::Mat<double> &md, int64_t c, const double *u, T *__restrict__ Wp,
-                                            double *__restrict__ pcell) {
-  double s[NEQ];
-  fk::cons2prim<NEQ>(md, u, s);
-#pragma unroll
-  for (int v = 0; v < NEQ; ++v) Wp[c * NEQ + v] = (T)s[v];
-  if constexpr (NEQ == 10) pcell[c] = s[0] * s[6] + (1 - s[0]) * s[7];
-}
-
-template <int NEQ, typename T>
-__global__ void __launch_bounds__(kBlock) prims(fk_params P, int64_t nc, const double *__restrict__ U,
-                                                T *__restrict__ Wp, double *__restrict__ pcell) {
-  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (c >= nc) return;
-  const fk::Mat<double> md = fk::make_mat<double>(P);
-  double u[NEQ];
-#pragma unroll
-  for (int v = 0; v < NEQ; ++v) u[v] = U[c * NEQ + v];
-  store_prims<NEQ, T>(md, c, u, Wp, pcell);
-}
-
-// RK stage (update.cu:19-103) from a stored residual, one cell per thread,
-// followed by the primitive states of the new state.
-template <int NEQ, typename T>
-__global__ void __launch_bounds__(kBlock) rk_update_prims(fk_params P, int64_t nc, const double *__restrict__ U,
-                                                          const double *__restrict__ U0, double *__restrict__ Unew,
-                                                          const double *__restrict__ rhs, const double *__restrict__ vol,
-                                                          int irk, double dt_val, const double *__restrict__ dt_ptr,
-                                                          T *__restrict__ Wp, double *__restrict__ pcell) {
-  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (c >= nc) return;
+// RK stage from a stored residual (update.cu:19-103).
+template <int NEQ>
+__global__ void __launch_bounds__(kBlock) rk_update(fk_params P, int64_t nc, const double *__restrict__ U,
+                                                    const double *__restrict__ U0, double *__restrict__ Unew,
+                                                    const double *__restrict__ rhs, const double *__restrict__ vol,
+                                                    int irk, double dt_val, const double *__restrict__ dt_ptr) {
+  const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nc * NEQ) return;
+  const int64_t c = i / NEQ;
+  const int v = (int)(i - c * NEQ);
   const double dt = dt_ptr ? *dt_ptr : dt_val;
   const double Vc = vol[c];
   constexpr int iz = NEQ == 5 ? 3 : 5;
-  double un[NEQ];
-#pragma unroll
-  for (int v = 0; v < NEQ; ++v) {
-    const int64_t i = c * NEQ + v;
-    const int neglect = (v == iz && P.neglectZ == 1);
-    double u;
-    if (P.rkSteps == 1) u = (1 - neglect) * (U[i] + dt * (1.0 / Vc) * rhs[i]);
-    else if (irk == 0) u = (1 - neglect) * (U0[i] + dt * (1.0 / Vc) * rhs[i]);
-    else if (irk == 1) u = (1 - neglect) * ((3.0 / 4.0) * U0[i] + (1.0 / 4.0) * U[i] + (1.0 / 4.0) * dt * (1.0 / Vc) * rhs[i]);
-    else u = (1 - neglect) * ((1.0 / 3.0) * U0[i] + (2.0 / 3.0) * U[i] + (2.0 / 3.0) * dt * (1.0 / Vc) * rhs[i]);
-    Unew[i] = u;
-    un[v] = u;
-  }
-  store_prims<NEQ, T>(fk::make_mat<double>(P), c, un, Wp, pcell);
+  const int neglect = (v == iz && P.neglectZ == 1);
+  double u;
+  if (P.rkSteps == 1) u = (1 - neglect) * (U[i] + dt * (1.0 / Vc) * rhs[i]);
+  else if (irk == 0) u = (1 - neglect) * (U0[i] + dt * (1.0 / Vc) * rhs[i]);
+  else if (irk == 1) u = (1 - neglect) * ((3.0 / 4.0) * U0[i] + (1.0 / 4.0) * U[i] + (1.0 / 4.0) * dt * (1.0 / Vc) * rhs[i]);
+  else u = (1 - neglect) * ((1.0 / 3.0) * U0[i] + (2.0 / 3.0) * U[i] + (2.0 / 3.0) * dt * (1.0 / Vc) * rhs[i]);
+  Unew[i] = u;
 }
 
 // min over cells of Vcfl/signal: per-block minima, then one block folds them
@@ -865,7 +841,7 @@ int upload_geometry(fk_ctx *c) {
 void free_all(fk_ctx *c) {
   void *ptrs[] = {c->nbr, c->mir, c->geo, c->rvec, c->wls, c->vol, c->volcfl, c->U[0], c->U[1], c->U[2], c->face,
                   c->gvel, c->rhs, c->dtc, c->red, c->dscal, c->dgrad, c->dtheta, c->lcount, c->scode, c->hoff,
-                  c->hcell, c->rlist, c->rpart, c->wprim, c->pcell};
+                  c->hcell, c->rlist, c->rpart};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->nbr = c->mir = nullptr;
@@ -874,9 +850,6 @@ void free_all(fk_ctx *c) {
   c->rlist = nullptr;
   c->rpart = nullptr;
   c->rlist_cap = 0;
-  c->wprim = nullptr;
-  c->pcell = nullptr;
-  c->w_valid = false;
   c->geo = c->rvec = c->wls = nullptr;
   c->vol = c->volcfl = nullptr;
   c->U[0] = c->U[1] = c->U[2] = nullptr;
@@ -903,10 +876,14 @@ int launch_pass_g_t(fk_ctx *c) {
       LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g<NEQ, T, VISC, HEAT>), dim3(gr), dim3(kBlock), lds,           \
                                                    c->stream, c->p, c->nc, U, c->scode, c->hoff, c->hcell, S, c->geo,   \
                                                    c->rvec, c->vol, face, gv, dg, dth));                                \
+    else if (c->roll_g)                                                                                                \
+      LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g_direct<NEQ, T, VISC, HEAT, 1>), dim3(gr), dim3(kBlock), 0,   \
+                                                   c->stream, c->p, c->nc, U, c->nbr, c->mir, c->geo, c->rvec, c->vol,  \
+                                                   face, gv, dg, dth));                                                 \
     else                                                                                                               \
-      LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g_direct<NEQ, T, VISC, HEAT>), dim3(gr), dim3(kBlock), 0,      \
-                                                   c->stream, c->p, c->nc, (const T *)c->wprim, c->nbr, c->mir, c->geo, \
-                                                   c->rvec, c->vol, face, gv, dg, dth));                                \
+      LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g_direct<NEQ, T, VISC, HEAT, 4>), dim3(gr), dim3(kBlock), 0,   \
+                                                   c->stream, c->p, c->nc, U, c->nbr, c->mir, c->geo, c->rvec, c->vol,  \
+                                                   face, gv, dg, dth));                                                 \
   } while (0)
   if (c->neq == 5) {
     if (heat) PG(5, true, true);
@@ -938,13 +915,11 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
     if (stage)                                                                                                         \
       LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, true>), dim3(gr), dim3(kBlock), lds,     \
                                              c->stream, c->p, c->nc, U, U0, Unew, c->nbr, c->mir, c->geo, wl, c->vol,   \
-                                             c->volcfl, face, gv, (const T *)c->wprim, c->pcell, c->rhs, c->dtc, irk,   \
-                                             dt, dtp));                                                                 \
+                                             c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp));                       \
     else                                                                                                               \
       LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false>), dim3(gr), dim3(kBlock), 0,      \
                                              c->stream, c->p, c->nc, U, U0, Unew, c->nbr, c->mir, c->geo, wl, c->vol,   \
-                                             c->volcfl, face, gv, (const T *)c->wprim, c->pcell, c->rhs, c->dtc, irk,   \
-                                             dt, dtp));                                                                 \
+                                             c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp));                       \
   } while (0)
   if (c->neq == 5) {
     if (heat) PF(5, true, true);
@@ -958,34 +933,11 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
   return FK_OK;
 }
 
-template <typename T>
-int launch_prims_t(fk_ctx *c) {
-  T *Wp = (T *)c->wprim;
-  if (c->neq == 5)
-    LAUNCH_TIMED(c, "prims", hipLaunchKernelGGL((prims<5, T>), dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream,
-                                                c->p, c->nc, c->U[c->ia], Wp, c->pcell));
-  else
-    LAUNCH_TIMED(c, "prims", hipLaunchKernelGGL((prims<10, T>), dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream,
-                                                c->p, c->nc, c->U[c->ia], Wp, c->pcell));
-  c->w_valid = true;
-  return FK_OK;
-}
-
-// primitive states of U[ia] for passes G and F, unless the kernel that wrote
-// U[ia] also wrote them
-int ensure_prims(fk_ctx *c) {
-  if (c->w_valid) return FK_OK;
-  return c->p.precision == FK_MIXED ? launch_prims_t<float>(c) : launch_prims_t<double>(c);
-}
-
 int do_pass_g(fk_ctx *c) {
-  int rc = ensure_prims(c);
-  if (rc) return rc;
   return c->p.precision == FK_MIXED ? launch_pass_g_t<float>(c) : launch_pass_g_t<double>(c);
 }
 
 int do_pass_f_rhs(fk_ctx *c) {
-  if (int rc0 = ensure_prims(c)) return rc0;
   int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 1>(c, 0, 0.0, nullptr, nullptr)
                                       : launch_pass_f_t<double, 1>(c, 0, 0.0, nullptr, nullptr);
   if (rc) return rc;
@@ -998,7 +950,6 @@ int do_update(fk_ctx *c, int irk, double dt, const double *dtp);
 // RK stage irk from pass F: fused (MODE 0) or residual (MODE 2) + rk_update,
 // as c->fuse_update selects; both give the same bits.
 int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp) {
-  if (int rc0 = ensure_prims(c)) return rc0;
   if (!c->fuse_update) {
     int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 2>(c, 0, 0.0, nullptr, nullptr)
                                         : launch_pass_f_t<double, 2>(c, 0, 0.0, nullptr, nullptr);
@@ -1011,36 +962,21 @@ int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp) {
   int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 0>(c, irk, dt, dtp, c->U[nb])
                                       : launch_pass_f_t<double, 0>(c, irk, dt, dtp, c->U[nb]);
   if (rc) return rc;
-  c->w_valid = false;
   c->ia = nb;
   c->rhs_valid = false;
   return FK_OK;
 }
 
-template <typename T>
-int launch_rk_prims_t(fk_ctx *c, int irk, double dt, const double *dtp, int nb) {
-  T *Wp = (T *)c->wprim;
-  const unsigned gr = grid_for(c->nc);
-  if (c->neq == 5)
-    LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update_prims<5, T>), dim3(gr), dim3(kBlock), 0, c->stream, c->p,
-                                                    c->nc, c->U[c->ia], c->U[c->iu0], c->U[nb], c->rhs, c->vol, irk,
-                                                    dt, dtp, Wp, c->pcell));
-  else
-    LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update_prims<10, T>), dim3(gr), dim3(kBlock), 0, c->stream,
-                                                    c->p, c->nc, c->U[c->ia], c->U[c->iu0], c->U[nb], c->rhs, c->vol,
-                                                    irk, dt, dtp, Wp, c->pcell));
-  return FK_OK;
-}
-
-// RK stage from the stored residual; also writes the primitive states of the
-// new state for the next pass G
 int do_update(fk_ctx *c, int irk, double dt, const double *dtp) {
   const int nb = next_buffer(c);
-  int rc = c->p.precision == FK_MIXED ? launch_rk_prims_t<float>(c, irk, dt, dtp, nb)
-                                      : launch_rk_prims_t<double>(c, irk, dt, dtp, nb);
-  if (rc) return rc;
+  const unsigned gr = grid_for(c->nc * c->neq);
+  if (c->neq == 5)
+    LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update<5>), dim3(gr), dim3(kBlock), 0, c->stream, c->p, c->nc,
+                                                    c->U[c->ia], c->U[c->iu0], c->U[nb], c->rhs, c->vol, irk, dt, dtp));
+  else
+    LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update<10>), dim3(gr), dim3(kBlock), 0, c->stream, c->p, c->nc,
+                                                    c->U[c->ia], c->U[c->iu0], c->U[nb], c->rhs, c->vol, irk, dt, dtp));
   c->ia = nb;
-  c->w_valid = true;
   return FK_OK;
 }
 
@@ -1088,7 +1024,6 @@ namespace {
 
 int do_relax(fk_ctx *c, fk_relax_stats *st, int mode = FK_RELAX_PLAIN) {
   if (c->p.model != FK_MODEL_TWO_PHASE) return fail(FK_EINVAL, "relaxation needs the two-phase model");
-  c->w_valid = false;  // the relaxation changes U in place
   unsigned long long *cnt = (unsigned long long *)c->lcount;
   if (mode == FK_RELAX_MLP || mode == FK_RELAX_COLD) {
     int rc = fk_relax_mlp_impl(c, mode == FK_RELAX_MLP);
@@ -1192,7 +1127,6 @@ int fk_set_params(fk_ctx *c, const fk_params *p) {
     if (rc) return rc;
   }
   c->rhs_valid = false;
-  c->w_valid = false;
   return FK_OK;
 }
 
@@ -1311,8 +1245,6 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
   for (int i = 0; i < 3; ++i) HIPCHK(hipMalloc(&c->U[i], cs * neq * 8));
   HIPCHK(hipMalloc(&c->face, (size_t)ns * neq * 8));
   HIPCHK(hipMalloc(&c->gvel, cs * 12 * 8));
-  HIPCHK(hipMalloc(&c->wprim, cs * neq * 8));
-  HIPCHK(hipMalloc(&c->pcell, cs * 8));
   HIPCHK(hipMalloc(&c->rhs, cs * neq * 8));
   HIPCHK(hipMalloc(&c->dtc, cs * 8));
   HIPCHK(hipMalloc(&c->red, 4096 * 8));
@@ -1347,7 +1279,6 @@ int fk_upload_state(fk_ctx *c, const double *U) {
   HIPCHK(hipMemcpy(c->U[0], U, (size_t)c->nc * c->neq * 8, hipMemcpyHostToDevice));
   c->have_state = true;
   c->rhs_valid = false;
-  c->w_valid = false;
   return FK_OK;
 }
 
@@ -1439,6 +1370,10 @@ int fk_set_tuning(fk_ctx *c, const char *name, int64_t value) {
   }
   if (std::strcmp(name, "stage_g") == 0) {
     c->stage_g = value != 0;
+    return FK_OK;
+  }
+  if (std::strcmp(name, "roll_g") == 0) {
+    c->roll_g = value != 0;
     return FK_OK;
   }
   return fail(FK_EINVAL, std::string("unknown tuning option ") + name);

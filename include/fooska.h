@@ -160,7 +160,9 @@ int fk_set_debug(fk_ctx *ctx, int on);
  * the face states and velocity gradients of its workgroup's cells to LDS and
  * reads in-workgroup mirror states from there.  "stage_g" (default 0): pass G
  * converts each workgroup's cells and halo cells to primitive variables once
- * into an LDS table (else every cell converts its 4 neighbours itself). */
+ * into an LDS table (else every cell converts its 4 neighbours itself).
+ * "roll_g" (default 0): pass G keeps its limiter and face-state loops rolled
+ * (fewer registers, less overlap). */
 int fk_set_tuning(fk_ctx *ctx, const char *name, int64_t value);
 
 /* ---- surrogate of the relaxation Newton solve (no counterpart in the
