@@ -62,6 +62,18 @@ def kernel_bytes(name, B, neq):
     }.get(name)
 
 
+def measured_traffic(kernel, cells):
+    """HBM bytes per launch of `kernel` (read + written) from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated
+    on a plain copy), or None when there is none for this kernel / size."""
+    path = os.path.join(ROOT, "profiles", "shock_bubble_48M_traffic.json")
+    if cells != 48_000_000 or not os.path.exists(path):
+        return None
+    with open(path) as f:
+        k = json.load(f)["kernels"].get(kernel)
+    return None if k is None else k["read_bytes_per_dispatch"] + k["write_bytes_per_dispatch"]
+
+
 def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None, fuse=0, stage_f=1, stage_g=0,
             roll_g=0):
     """Time `steps` explicitLoop steps on the device (state resident in HBM).
@@ -196,6 +208,7 @@ def main():
     avg_f = ms_f / max(n_f, 1) / 1e3
     bpc = kernel_bytes(dom, B, 10)
     achieved = bpc * C / avg_f if avg_f > 0 else 0.0
+    traffic = measured_traffic(dom, C)
     whole = (B["total"] * C * s * args.steps + relax_bytes(10) * C * args.steps) / mixed["el"]
     out = {
         "metric": "cell-RK-stage updates/s, 1 MI355X: mixed+surrogate vs FP64; % of HBM peak",
@@ -218,7 +231,7 @@ def main():
                    "stage_f": args.stage_f, "stage_g": args.stage_g,
                    "roll_g": args.roll_g},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK, "traffic": None,
+                     "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": dom, "bytes_per_cell": bpc, "avg_launch_ms": avg_f * 1e3},
         "whole_step_hbm_frac": whole / HBM_PEAK,
         "stream_copy": copy,

@@ -437,7 +437,8 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   const fk::Mat<T> mt = fk::make_mat<T>(P);
   constexpr int iu = Model<NEQ>::iu;
   constexpr int NG = ngrad<NEQ, HEAT>();
-  const double *Uc = U + c * NEQ;
+  double Uc[NEQ];
+  fk::load_rec<NEQ>(U + c * NEQ, Uc);
   // side terms of this cell (fluxSource / computeCenterPress, two-phase)
   T avf = 0, pLiq = 0, pGas = 0;
   double pCd = 0;
@@ -704,9 +705,8 @@ __global__ void __launch_bounds__(kBlock) relax_therm(fk_params P, int64_t nc, d
   const int l = threadIdx.x & 63;
   int chem = 0, it = 0, ts = 0;
   if (c < nc) {
-    double u[9];
-#pragma unroll
-    for (int v = 0; v < 9; ++v) u[v] = U[c * 10 + v];
+    double u[10];
+    fk::load_rec<10>(U + c * 10, u);
     if (P.chemFlag == 1) {
       const double avf = clampd(P, u[0]);
       const double rho = u[1] / avf;
@@ -744,8 +744,7 @@ __global__ void __launch_bounds__(kBlock) relax_chem(fk_params P, double *__rest
   for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBlock) {
     const int64_t c = list[e];
     double u[10];
-#pragma unroll
-    for (int v = 0; v < 10; ++v) u[v] = U[c * 10 + v];
+    fk::load_rec<10>(U + c * 10, u);
     nit += chem_dev(P, u);
     U[c * 10 + 0] = u[0];
     U[c * 10 + 1] = u[1];

@@ -391,9 +391,8 @@ __global__ void __launch_bounds__(256) relax_screen_k(fk_params P, int64_t nc, d
   const int l = threadIdx.x & 63;
   int chem = 0, it = 0, push = 0;
   if (c < nc) {
-    double u[9];
-#pragma unroll
-    for (int v = 0; v < 9; ++v) u[v] = U[c * 10 + v];
+    double u[10];
+    fk::load_rec<10>(U + c * 10, u);
     if (P.chemFlag == 1) {
       const double avf = fk::clampd(P, u[0]);
       const double rho = u[1] / avf, enr = u[6] / (avf * rho);
