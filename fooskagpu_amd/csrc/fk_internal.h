@@ -94,6 +94,7 @@ struct fk_ctx {
   bool stage_f = true;       // pass F stages the workgroup's face states in LDS (mixed only)
   bool stage_g = false;      // pass G stages neighbour primitive states in LDS
   bool roll_g = false;       // pass G limiter / output face loops not unrolled
+  bool stage_u = false;      // pass G copies its workgroup's conserved states to LDS
   fkx::Timer tm;
   fkx::Mlp mlp;
   // host geometry kept to rebuild the precision-dependent arrays

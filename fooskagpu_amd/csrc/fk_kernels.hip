@@ -271,19 +271,39 @@ __global__ void __launch_bounds__(kBlock) pass_g(fk_params P, int64_t nc, const 
 // Pass G without LDS staging: every cell converts its 4 neighbours itself
 // (5 cons2prim per cell) and keeps all variables in registers; full face
 // records are written once.  Same arithmetic and order as pass_g.
-template <int NEQ, typename T, bool VISC, bool HEAT, int UNR>
+//
+// SU: the workgroup first copies the conserved states of its own cells into
+// LDS (coalesced, 16-byte words); cons2prim of the cell and of neighbours
+// inside the workgroup then reads LDS, of other neighbours global memory.
+template <int NEQ, typename T, bool VISC, bool HEAT, int UNR, bool SU>
 __global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc, const double *__restrict__ U,
                                                  const int32_t *__restrict__ nbr, const int32_t *__restrict__ mir,
                                                  const void *__restrict__ geo, const void *__restrict__ rvec,
                                                  const double *__restrict__ vol, T *__restrict__ face,
                                                  T *__restrict__ gvel, double *__restrict__ dgrad,
                                                  double *__restrict__ dtheta) {
-  const int64_t c = xcd_block() * kBlock + threadIdx.x;
+  const int64_t c0 = xcd_block() * kBlock;
+  const int64_t c = c0 + threadIdx.x;
+  __shared__ __attribute__((aligned(16))) double sU[SU ? kBlock * NEQ : 2];
+  if constexpr (SU) {
+    const int64_t ncell = (nc - c0) < kBlock ? (nc - c0) : kBlock;
+    const int64_t nw = ncell * NEQ * 8 / 16;  // NEQ * 8 bytes per cell: 40 or 80, a multiple of 16 per 2 cells
+    const uint4 *src = reinterpret_cast<const uint4 *>(U + c0 * NEQ);
+    uint4 *dst = reinterpret_cast<uint4 *>(sU);
+    for (int64_t i = threadIdx.x; i < nw; i += kBlock) dst[i] = src[i];
+    if ((ncell * NEQ) % 2 == 1 && threadIdx.x == 0) sU[ncell * NEQ - 1] = U[(c0 + ncell) * NEQ - 1];
+    __syncthreads();
+  }
   if (c >= nc) return;
+  // conserved state of `cell`, from LDS when staged
+  auto urec = [&](int64_t cell) -> const double * {
+    if (SU && cell >= c0 && cell < c0 + kBlock) return sU + (cell - c0) * NEQ;
+    return U + cell * NEQ;
+  };
   const fk::Mat<double> md = fk::make_mat<double>(P);
   const fk::Mat<T> mt = fk::make_mat<T>(P);
   double Wd[NEQ];
-  fk::cons2prim<NEQ>(md, U + c * NEQ, Wd);
+  fk::cons2prim<NEQ>(md, urec(c), Wd);
   T W[NEQ];
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) W[v] = (T)Wd[v];
@@ -305,7 +325,7 @@ __global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc,
     T Wn[NEQ];
     if (nb >= 0) {
       double t[NEQ];
-      fk::cons2prim<NEQ>(md, U + (int64_t)nb * NEQ, t);
+      fk::cons2prim<NEQ>(md, urec(nb), t);
 #pragma unroll
       for (int v = 0; v < NEQ; ++v) Wn[v] = (T)t[v];
     } else {
@@ -411,15 +431,20 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   T *sF = reinterpret_cast<T *>(lds_raw);
   T *sG = sF + kBlock * 4 * NEQ;
   if constexpr (STAGE) {
-    if (c < nc) {
-      const T *src = face + c * 4 * NEQ;
-      T *dst = sF + threadIdx.x * 4 * NEQ;
-#pragma unroll
-      for (int v = 0; v < 4 * NEQ; ++v) dst[v] = src[v];
-      if constexpr (VISC) {
-#pragma unroll
-        for (int v = 0; v < NGS; ++v) sG[threadIdx.x * NGS + v] = gvel[c * NGS + v];
-      }
+    // the workgroup's face records and gradients are contiguous in memory:
+    // copy them with consecutive threads on consecutive 16-byte words
+    // (coalesced), not record by record
+    const int64_t ncell = (nc - c0) < kBlock ? (nc - c0) : kBlock;
+    {
+      const int64_t nw = ncell * 4 * NEQ * (int64_t)sizeof(T) / 16;  // 16-byte words (4 NEQ sizeof(T) = 160 or 320)
+      const uint4 *src = reinterpret_cast<const uint4 *>(face + c0 * 4 * NEQ);
+      uint4 *dst = reinterpret_cast<uint4 *>(sF);
+      for (int64_t i = threadIdx.x; i < nw; i += kBlock) dst[i] = src[i];
+    }
+    if constexpr (VISC) {
+      const int64_t ng = ncell * NGS;
+      const T *src = gvel + c0 * NGS;
+      for (int64_t i = threadIdx.x; i < ng; i += kBlock) sG[i] = src[i];
     }
     __syncthreads();
   }
@@ -876,13 +901,17 @@ int launch_pass_g_t(fk_ctx *c) {
                                                    c->stream, c->p, c->nc, U, c->scode, c->hoff, c->hcell, S, c->geo,   \
                                                    c->rvec, c->vol, face, gv, dg, dth));                                \
     else if (c->roll_g)                                                                                                \
-      LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g_direct<NEQ, T, VISC, HEAT, 1>), dim3(gr), dim3(kBlock), 0,   \
-                                                   c->stream, c->p, c->nc, U, c->nbr, c->mir, c->geo, c->rvec, c->vol,  \
-                                                   face, gv, dg, dth));                                                 \
+      LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g_direct<NEQ, T, VISC, HEAT, 1, false>), dim3(gr),             \
+                                                   dim3(kBlock), 0, c->stream, c->p, c->nc, U, c->nbr, c->mir, c->geo,  \
+                                                   c->rvec, c->vol, face, gv, dg, dth));                                \
+    else if (c->stage_u)                                                                                               \
+      LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g_direct<NEQ, T, VISC, HEAT, 4, true>), dim3(gr),              \
+                                                   dim3(kBlock), 0, c->stream, c->p, c->nc, U, c->nbr, c->mir, c->geo,  \
+                                                   c->rvec, c->vol, face, gv, dg, dth));                                \
     else                                                                                                               \
-      LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g_direct<NEQ, T, VISC, HEAT, 4>), dim3(gr), dim3(kBlock), 0,   \
-                                                   c->stream, c->p, c->nc, U, c->nbr, c->mir, c->geo, c->rvec, c->vol,  \
-                                                   face, gv, dg, dth));                                                 \
+      LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g_direct<NEQ, T, VISC, HEAT, 4, false>), dim3(gr),             \
+                                                   dim3(kBlock), 0, c->stream, c->p, c->nc, U, c->nbr, c->mir, c->geo,  \
+                                                   c->rvec, c->vol, face, gv, dg, dth));                                \
   } while (0)
   if (c->neq == 5) {
     if (heat) PG(5, true, true);
@@ -1373,6 +1402,10 @@ int fk_set_tuning(fk_ctx *c, const char *name, int64_t value) {
   }
   if (std::strcmp(name, "roll_g") == 0) {
     c->roll_g = value != 0;
+    return FK_OK;
+  }
+  if (std::strcmp(name, "stage_u") == 0) {
+    c->stage_u = value != 0;
     return FK_OK;
   }
   return fail(FK_EINVAL, std::string("unknown tuning option ") + name);

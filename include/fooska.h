@@ -162,7 +162,9 @@ int fk_set_debug(fk_ctx *ctx, int on);
  * converts each workgroup's cells and halo cells to primitive variables once
  * into an LDS table (else every cell converts its 4 neighbours itself).
  * "roll_g" (default 0): pass G keeps its limiter and face-state loops rolled
- * (fewer registers, less overlap). */
+ * (fewer registers, less overlap).  "stage_u" (default 0): pass G copies the
+ * conserved states of its workgroup's cells to LDS and reads in-workgroup
+ * neighbours from there. */
 int fk_set_tuning(fk_ctx *ctx, const char *name, int64_t value);
 
 /* ---- surrogate of the relaxation Newton solve (no counterpart in the

@@ -183,12 +183,14 @@ def test_tuning_options_do_not_change_results(gpu, precision):
     c = SMALL["shock_bubble"]()
     p = dict(c.params, precision=precision)
     out = []
-    for fuse, stage_f, stage_g, roll_g in ((0, 0, 0, 0), (1, 0, 0, 1), (0, 1, 0, 0), (1, 1, 1, 0), (0, 0, 1, 1)):
+    for fuse, stage_f, stage_g, roll_g, stage_u in ((0, 0, 0, 0, 0), (1, 0, 0, 1, 0), (0, 1, 0, 0, 1),
+                                                    (1, 1, 1, 0, 0), (0, 0, 1, 1, 0), (1, 1, 0, 0, 1)):
         with GpuSolver(p, c.mesh, c.U) as g:
             g.set_tuning("fuse_update", fuse)
             g.set_tuning("stage_f", stage_f)
             g.set_tuning("stage_g", stage_g)
             g.set_tuning("roll_g", roll_g)
+            g.set_tuning("stage_u", stage_u)
             g.run_steps(3)
             out.append(g.download_state())
     for o in out[1:]:
