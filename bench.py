@@ -74,8 +74,7 @@ def measured_traffic(kernel, cells):
     return None if k is None else k["read_bytes_per_dispatch"] + k["write_bytes_per_dispatch"]
 
 
-def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None, fuse=0, stage_f=1, stage_g=0,
-            roll_g=0, stage_u=0):
+def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None, fuse=0, stage_f=1):
     """Time `steps` explicitLoop steps on the device (state resident in HBM).
     train: dict of surrogate-training options (done before the timed region)."""
     from fooskagpu_amd import GpuSolver, surrogate
@@ -83,9 +82,6 @@ def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None, 
     g = GpuSolver(p, case.mesh, case.U, device=device)
     g.set_tuning("fuse_update", fuse)
     g.set_tuning("stage_f", stage_f)
-    g.set_tuning("stage_g", stage_g)
-    g.set_tuning("roll_g", roll_g)
-    g.set_tuning("stage_u", stage_u)
     info = {}
     if train is not None:
         t0 = time.perf_counter()
@@ -149,12 +145,6 @@ def main():
     ap.add_argument("--no-mlp", action="store_true", help="skip the surrogate run")
     ap.add_argument("--order", default="morton", choices=["rcm", "morton", "none"], help="cell order of the mesh")
     ap.add_argument("--train-steps", type=int, default=3000)
-    ap.add_argument("--stage-u", type=int, default=0, choices=[0, 1],
-                    help="pass G copies its workgroup's conserved states to LDS; same results")
-    ap.add_argument("--roll-g", type=int, default=0, choices=[0, 1],
-                    help="pass G keeps its limiter / face-state loops rolled; same results")
-    ap.add_argument("--stage-g", type=int, default=0, choices=[0, 1],
-                    help="pass G stages neighbour primitive states in LDS; same results")
     ap.add_argument("--stage-f", type=int, default=1, choices=[0, 1],
                     help="pass F stages the workgroup's face states in LDS (mixed precision); same results")
     ap.add_argument("--fuse", type=int, default=0, choices=[0, 1],
@@ -192,8 +182,7 @@ def main():
             continue
         log(f"{name}: warmup {args.warmup}, timing {args.steps} steps")
         el, kt, U, dt, info = run_gpu(case, prec, args.steps, args.warmup, relax_mode=mode, train=tr, fuse=args.fuse,
-                                      stage_f=args.stage_f, stage_g=args.stage_g,
-                                      roll_g=args.roll_g, stage_u=args.stage_u)
+                                      stage_f=args.stage_f)
         rate = C * s * args.steps / el
         log(f"{name}: {el:.3f} s, {rate:.4g} cell-stage/s, last dt {dt:.3e}, finite={bool(np.isfinite(U).all())}")
         for k, (n, ms) in sorted(kt.items(), key=lambda kv: -kv[1][1]):
@@ -231,8 +220,7 @@ def main():
                                + ("MLP warm start + FP64 Newton polish" if head == "mixed_mlp" else
                                   "plain Newton relaxation"),
                    "cells": C, "faces": case.mesh.n_faces, "cell_order": args.order, "fuse_update": args.fuse,
-                   "stage_f": args.stage_f, "stage_g": args.stage_g,
-                   "roll_g": args.roll_g, "stage_u": args.stage_u},
+                   "stage_f": args.stage_f},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": dom, "bytes_per_cell": bpc, "avg_launch_ms": avg_f * 1e3},

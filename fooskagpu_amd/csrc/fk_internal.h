@@ -65,9 +65,6 @@ struct fk_ctx {
   bool have_mesh = false, have_state = false, debug = false, timing = false;
   // mesh
   int32_t *nbr = nullptr, *mir = nullptr;
-  // pass G neighbour staging: slot codes, per-workgroup halo lists, table size
-  int32_t *scode = nullptr, *hoff = nullptr, *hcell = nullptr;
-  int halo_tab = 0;
   void *geo = nullptr, *rvec = nullptr, *wls = nullptr;  // T-typed, sized for double
   double *vol = nullptr, *volcfl = nullptr;
   std::vector<int32_t> slot_face;  // host copy of slot -> face id
@@ -92,9 +89,6 @@ struct fk_ctx {
   bool rhs_valid = false;
   bool fuse_update = false;  // RK update inside pass F (else residual + rk_update)
   bool stage_f = true;       // pass F stages the workgroup's face states in LDS (mixed only)
-  bool stage_g = false;      // pass G stages neighbour primitive states in LDS
-  bool roll_g = false;       // pass G limiter / output face loops not unrolled
-  bool stage_u = false;      // pass G copies its workgroup's conserved states to LDS
   fkx::Timer tm;
   fkx::Mlp mlp;
   // host geometry kept to rebuild the precision-dependent arrays

@@ -62,21 +62,9 @@ constexpr int ngrad() {
 // [x G/8, (x+1) G/8): neighbouring cells (close in the mesh order) then share
 // an L2.  G is a multiple of 8 (padded by grid_xcd).
 constexpr int kXcd = 8;
-// halo cells per workgroup staged in LDS by pass G (more are converted in place)
-constexpr int kHaloCap = 512;
 __device__ __forceinline__ int64_t xcd_block() {
   const int64_t b = blockIdx.x, per = gridDim.x / kXcd;
   return (b % kXcd) * per + b / kXcd;
-}
-
-// cons2prim of a neighbour outside the LDS table (rare): kept out of line so
-// that the face loop of pass G is not register-allocated for it
-template <int NEQ, typename T>
-__device__ __noinline__ void prim_out_of_line(const fk::Mat<double> md, const double *__restrict__ Uc, T *Wn) {
-  double t[NEQ];
-  fk::cons2prim<NEQ>(md, Uc, t);
-#pragma unroll
-  for (int v = 0; v < NEQ; ++v) Wn[v] = (T)t[v];
 }
 
 template <typename T>
@@ -84,226 +72,24 @@ __device__ __forceinline__ V4<T> ld4(const void *p, int64_t i) {
   return reinterpret_cast<const V4<T> *>(p)[i];
 }
 
-// Face loop of pass G for the variables [V0, V1): Green-Gauss gradient
-// (owner + / neighbour -, 0.5 (W + W_n) n A / V per face, in slot order),
-// neighbour min/max, limiter and MUSCL face states, velocity gradient.
-template <int NEQ, typename T, bool VISC, bool HEAT, int V0, int V1>
-__device__ __forceinline__ void pass_g_group(const fk_params &P, const fk::Mat<double> &md, const fk::Mat<T> &mt,
-                                             int64_t c, const double *__restrict__ U,
-                                             const int32_t *__restrict__ code, const T *sW, int S,
-                                             const void *__restrict__ geo, const void *__restrict__ rvec, T V,
-                                             const T (&W)[NEQ], T *__restrict__ face, T *__restrict__ gvel,
-                                             double *__restrict__ dgrad, double *__restrict__ dtheta) {
-  constexpr int NV = V1 - V0;
-  T g[NV][3], mn[NV], mx[NV];
-#pragma unroll
-  for (int i = 0; i < NV; ++i) {
-    g[i][0] = g[i][1] = g[i][2] = 0;
-    mn[i] = (T)1e12;
-    mx[i] = (T)-1e12;
-  }
-#pragma unroll 1
-  for (int j = 0; j < 4; ++j) {
-    const int64_t s = c * 4 + j;
-    const int cd = code[s];
-    const V4<T> ga = ld4<T>(geo, s);
-    const T n[3] = {ga.x, ga.y, ga.z};
-    T Wn[NV];
-    bool own;
-    if (cd >= 0) {
-      own = cd & 1;
-      const int e = cd >> 1;
-#pragma unroll
-      for (int i = 0; i < NV; ++i) Wn[i] = sW[(V0 + i) * S + e];
-    } else {
-      T full[NEQ];
-      if (cd > -8) {
-        own = true;
-#pragma unroll
-        for (int v = 0; v < NEQ; ++v) full[v] = W[v];
-        fk::ghost<NEQ, T>(mt, -cd, n, full);
-      } else {
-        const int x = -cd - 8;
-        own = x & 1;
-        prim_out_of_line<NEQ, T>(md, U + (int64_t)(x >> 1) * NEQ, full);
-      }
-#pragma unroll
-      for (int i = 0; i < NV; ++i) Wn[i] = full[V0 + i];
-    }
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      const T o = Wn[i];
-      if (!(mx[i] >= o)) mx[i] = o;
-      if (!(mn[i] <= o)) mn[i] = o;
-      const T avg = (W[V0 + i] + Wn[i]) / 2;
-      if (own) {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) g[i][d] += avg * n[d] * ga.w / V;
-      } else {
-#pragma unroll
-        for (int d = 0; d < 3; ++d) g[i][d] += -avg * n[d] * ga.w / V;
-      }
-    }
-  }
-  // limiter: theta = min over faces of clamp(phi, 0, 1)
-  T th[NV];
-#pragma unroll
-  for (int i = 0; i < NV; ++i) th[i] = (T)1e9;
-#pragma unroll 1
-  for (int j = 0; j < 4; ++j) {
-    const V4<T> r = ld4<T>(rvec, c * 4 + j);
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      const T w0 = W[V0 + i];
-      const T wu = w0 + r.x * g[i][0] + r.y * g[i][1] + r.z * g[i][2];
-      T phi;
-      if (wu - w0 != 0)
-        phi = (wu > w0) ? (mx[i] - w0) / (2 * (wu - w0)) : (mn[i] - w0) / (2 * (wu - w0));
-      else
-        phi = 1;
-      const T tt = fmax((T)0, fmin(phi, (T)1));
-      if (!(th[i] <= tt)) th[i] = tt;
-    }
-  }
-  // MUSCL face states
-#pragma unroll 1
-  for (int j = 0; j < 4; ++j) {
-    const V4<T> r = ld4<T>(rvec, c * 4 + j);
-    T *out = face + (c * 4 + j) * NEQ + V0;
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      const T ext = r.x * g[i][0] + r.y * g[i][1] + r.z * g[i][2];
-      out[i] = W[V0 + i] + th[i] * ext;
-    }
-  }
-  if constexpr (VISC) {
-    constexpr int iu = Model<NEQ>::iu;
-    constexpr int NG = ngrad<NEQ, HEAT>();
-    T *G = gvel + c * NG;
-    if constexpr (iu >= V0 && iu + 3 <= V1) {
-#pragma unroll
-      for (int i = 0; i < 3; ++i)
-#pragma unroll
-        for (int d = 0; d < 3; ++d) G[i * 3 + d] = g[iu - V0 + i][d];
-    }
-    if constexpr (HEAT && V0 == 0 && V1 >= 5) {
-      const T gm = (T)P.gamma[0], pi = (T)P.pInf[0];
-      const T R = (T)P.Cp[0] * (gm - 1) / gm;
-#pragma unroll
-      for (int d = 0; d < 3; ++d) G[9 + d] = (g[4][d] - ((W[4] + pi) / W[0]) * g[0][d]) / (W[0] * R);
-    }
-  }
-  if (dgrad) {
-#pragma unroll
-    for (int i = 0; i < NV; ++i) {
-      dtheta[c * NEQ + V0 + i] = (double)th[i];
-#pragma unroll
-      for (int d = 0; d < 3; ++d) dgrad[(c * NEQ + V0 + i) * 3 + d] = (double)g[i][d];
-    }
-  }
-}
-
 // Pass G: first-order states, Green-Gauss gradient, neighbour min/max,
 // limiter and MUSCL face states (computePrimVec.cu:38-80,
-// computeGrad.cu:51-91, computeFaceValues.cu:29-113).
-//
-// The workgroup stages the primitive states it needs in LDS first: its own
-// kBlock cells, then the halo cells (neighbours outside the workgroup, listed
-// per workgroup on the host, hcell[hoff[b] .. hoff[b+1])), each converted once
-// by cons2prim.  The face loop then reads neighbour states from LDS through
-// the slot code (slot codes are built in fk_upload_mesh):
-//   code >= 0        (table index << 1) | owner flag
-//   -5 <= code <= -1 boundary face with bc = -code
-//   code <= -8       neighbour not in the table (halo beyond kHaloCap):
-//                    -code - 8 = (cell << 1) | owner flag, converted here
-// LDS layout: SoA sW[v * S + e], S = kBlock + table halo size.
+// computeGrad.cu:51-91, computeFaceValues.cu:29-113).  Every cell converts
+// its own state and its 4 neighbours' (cons2prim, FP64) and keeps all
+// variables in registers; full face records are written once.
 template <int NEQ, typename T, bool VISC, bool HEAT>
 __global__ void __launch_bounds__(kBlock) pass_g(fk_params P, int64_t nc, const double *__restrict__ U,
-                                                 const int32_t *__restrict__ code, const int32_t *__restrict__ hoff,
-                                                 const int32_t *__restrict__ hcell, int S,
-                                                 const void *__restrict__ geo, const void *__restrict__ rvec,
-                                                 const double *__restrict__ vol, T *__restrict__ face,
-                                                 T *__restrict__ gvel, double *__restrict__ dgrad,
-                                                 double *__restrict__ dtheta) {
-  extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
-  T *sW = reinterpret_cast<T *>(lds_raw);
-  const int64_t lb = xcd_block();
-  const int64_t c = lb * kBlock + threadIdx.x;
-  const bool live = c < nc;
-  const fk::Mat<double> md = fk::make_mat<double>(P);
-  const fk::Mat<T> mt = fk::make_mat<T>(P);
-  T W[NEQ];
-  if (live) {
-    double Wd[NEQ];
-    fk::cons2prim<NEQ>(md, U + c * NEQ, Wd);
-#pragma unroll
-    for (int v = 0; v < NEQ; ++v) {
-      W[v] = (T)Wd[v];
-      sW[v * S + threadIdx.x] = W[v];
-    }
-  }
-  {
-    const int h0 = hoff[lb], h1 = hoff[lb + 1];
-    for (int h = h0 + (int)threadIdx.x; h < h1; h += kBlock) {
-      double t[NEQ];
-      fk::cons2prim<NEQ>(md, U + (int64_t)hcell[h] * NEQ, t);
-#pragma unroll
-      for (int v = 0; v < NEQ; ++v) sW[v * S + kBlock + (h - h0)] = (T)t[v];
-    }
-  }
-  __syncthreads();
-  if (!live) return;
-  const T V = (T)vol[c];
-  // variables in groups (velocity components in one group), so that the
-  // gradient, min/max and limiter state of a group fits in registers
-  if constexpr (NEQ == 10 && sizeof(T) == 8) {
-    pass_g_group<NEQ, T, VISC, HEAT, 0, 3>(P, md, mt, c, U, code, sW, S, geo, rvec, V, W, face, gvel, dgrad, dtheta);
-    pass_g_group<NEQ, T, VISC, HEAT, 3, 6>(P, md, mt, c, U, code, sW, S, geo, rvec, V, W, face, gvel, dgrad, dtheta);
-    pass_g_group<NEQ, T, VISC, HEAT, 6, 10>(P, md, mt, c, U, code, sW, S, geo, rvec, V, W, face, gvel, dgrad, dtheta);
-  } else if constexpr (NEQ == 10) {
-    pass_g_group<NEQ, T, VISC, HEAT, 0, 6>(P, md, mt, c, U, code, sW, S, geo, rvec, V, W, face, gvel, dgrad, dtheta);
-    pass_g_group<NEQ, T, VISC, HEAT, 6, 10>(P, md, mt, c, U, code, sW, S, geo, rvec, V, W, face, gvel, dgrad, dtheta);
-  } else {
-    pass_g_group<NEQ, T, VISC, HEAT, 0, NEQ>(P, md, mt, c, U, code, sW, S, geo, rvec, V, W, face, gvel, dgrad, dtheta);
-  }
-}
-
-// Pass G without LDS staging: every cell converts its 4 neighbours itself
-// (5 cons2prim per cell) and keeps all variables in registers; full face
-// records are written once.  Same arithmetic and order as pass_g.
-//
-// SU: the workgroup first copies the conserved states of its own cells into
-// LDS (coalesced, 16-byte words); cons2prim of the cell and of neighbours
-// inside the workgroup then reads LDS, of other neighbours global memory.
-template <int NEQ, typename T, bool VISC, bool HEAT, int UNR, bool SU>
-__global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc, const double *__restrict__ U,
                                                  const int32_t *__restrict__ nbr, const int32_t *__restrict__ mir,
                                                  const void *__restrict__ geo, const void *__restrict__ rvec,
                                                  const double *__restrict__ vol, T *__restrict__ face,
                                                  T *__restrict__ gvel, double *__restrict__ dgrad,
                                                  double *__restrict__ dtheta) {
-  const int64_t c0 = xcd_block() * kBlock;
-  const int64_t c = c0 + threadIdx.x;
-  __shared__ __attribute__((aligned(16))) double sU[SU ? kBlock * NEQ : 2];
-  if constexpr (SU) {
-    const int64_t ncell = (nc - c0) < kBlock ? (nc - c0) : kBlock;
-    const int64_t nw = ncell * NEQ * 8 / 16;  // NEQ * 8 bytes per cell: 40 or 80, a multiple of 16 per 2 cells
-    const uint4 *src = reinterpret_cast<const uint4 *>(U + c0 * NEQ);
-    uint4 *dst = reinterpret_cast<uint4 *>(sU);
-    for (int64_t i = threadIdx.x; i < nw; i += kBlock) dst[i] = src[i];
-    if ((ncell * NEQ) % 2 == 1 && threadIdx.x == 0) sU[ncell * NEQ - 1] = U[(c0 + ncell) * NEQ - 1];
-    __syncthreads();
-  }
+  const int64_t c = xcd_block() * kBlock + threadIdx.x;
   if (c >= nc) return;
-  // conserved state of `cell`, from LDS when staged
-  auto urec = [&](int64_t cell) -> const double * {
-    if (SU && cell >= c0 && cell < c0 + kBlock) return sU + (cell - c0) * NEQ;
-    return U + cell * NEQ;
-  };
   const fk::Mat<double> md = fk::make_mat<double>(P);
   const fk::Mat<T> mt = fk::make_mat<T>(P);
   double Wd[NEQ];
-  fk::cons2prim<NEQ>(md, urec(c), Wd);
+  fk::cons2prim<NEQ>(md, U + c * NEQ, Wd);
   T W[NEQ];
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) W[v] = (T)Wd[v];
@@ -325,7 +111,7 @@ __global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc,
     T Wn[NEQ];
     if (nb >= 0) {
       double t[NEQ];
-      fk::cons2prim<NEQ>(md, urec(nb), t);
+      fk::cons2prim<NEQ>(md, U + (int64_t)nb * NEQ, t);
 #pragma unroll
       for (int v = 0; v < NEQ; ++v) Wn[v] = (T)t[v];
     } else {
@@ -348,12 +134,11 @@ __global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc,
       }
     }
   }
-  // limiter: theta = min over faces of clamp(phi, 0, 1); UNR = unroll factor
-  // of the face loops below (1: lower register use, 4: more overlap)
+  // limiter: theta = min over faces of clamp(phi, 0, 1)
   T th[NEQ];
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) th[v] = (T)1e9;
-#pragma unroll UNR
+#pragma unroll
   for (int j = 0; j < 4; ++j) {
     const V4<T> r = ld4<T>(rvec, c * 4 + j);
 #pragma unroll
@@ -369,7 +154,7 @@ __global__ void __launch_bounds__(kBlock) pass_g_direct(fk_params P, int64_t nc,
     }
   }
   // MUSCL face states
-#pragma unroll UNR
+#pragma unroll
   for (int j = 0; j < 4; ++j) {
     const V4<T> r = ld4<T>(rvec, c * 4 + j);
     T *out = face + (c * 4 + j) * NEQ;
@@ -431,20 +216,17 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   T *sF = reinterpret_cast<T *>(lds_raw);
   T *sG = sF + kBlock * 4 * NEQ;
   if constexpr (STAGE) {
-    // the workgroup's face records and gradients are contiguous in memory:
-    // copy them with consecutive threads on consecutive 16-byte words
-    // (coalesced), not record by record
-    const int64_t ncell = (nc - c0) < kBlock ? (nc - c0) : kBlock;
-    {
-      const int64_t nw = ncell * 4 * NEQ * (int64_t)sizeof(T) / 16;  // 16-byte words (4 NEQ sizeof(T) = 160 or 320)
-      const uint4 *src = reinterpret_cast<const uint4 *>(face + c0 * 4 * NEQ);
-      uint4 *dst = reinterpret_cast<uint4 *>(sF);
-      for (int64_t i = threadIdx.x; i < nw; i += kBlock) dst[i] = src[i];
-    }
-    if constexpr (VISC) {
-      const int64_t ng = ncell * NGS;
-      const T *src = gvel + c0 * NGS;
-      for (int64_t i = threadIdx.x; i < ng; i += kBlock) sG[i] = src[i];
+    // each thread copies its own cell's records (measured faster here than a
+    // coalesced copy of the workgroup's contiguous block: 6.7 vs 7.5 ms)
+    if (c < nc) {
+      const T *src = face + c * 4 * NEQ;
+      T *dst = sF + threadIdx.x * 4 * NEQ;
+#pragma unroll
+      for (int v = 0; v < 4 * NEQ; ++v) dst[v] = src[v];
+      if constexpr (VISC) {
+#pragma unroll
+        for (int v = 0; v < NGS; ++v) sG[threadIdx.x * NGS + v] = gvel[c * NGS + v];
+      }
     }
     __syncthreads();
   }
@@ -864,13 +646,11 @@ int upload_geometry(fk_ctx *c) {
 
 void free_all(fk_ctx *c) {
   void *ptrs[] = {c->nbr, c->mir, c->geo, c->rvec, c->wls, c->vol, c->volcfl, c->U[0], c->U[1], c->U[2], c->face,
-                  c->gvel, c->rhs, c->dtc, c->red, c->dscal, c->dgrad, c->dtheta, c->lcount, c->scode, c->hoff,
-                  c->hcell, c->rlist, c->rpart};
+                  c->gvel, c->rhs, c->dtc, c->red, c->dscal, c->dgrad, c->dtheta, c->lcount, c->rlist,
+                  c->rpart};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->nbr = c->mir = nullptr;
-  c->scode = c->hoff = c->hcell = nullptr;
-  c->halo_tab = 0;
   c->rlist = nullptr;
   c->rpart = nullptr;
   c->rlist_cap = 0;
@@ -892,27 +672,10 @@ int launch_pass_g_t(fk_ctx *c) {
   const double *U = c->U[c->ia];
   T *face = (T *)c->face;
   T *gv = (T *)c->gvel;
-  const int S = kBlock + c->halo_tab;
-  const size_t lds = (size_t)S * c->neq * sizeof(T);
-#define PG(NEQ, VISC, HEAT)                                                                                            \
-  do {                                                                                                                 \
-    if (c->stage_g)                                                                                                    \
-      LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g<NEQ, T, VISC, HEAT>), dim3(gr), dim3(kBlock), lds,           \
-                                                   c->stream, c->p, c->nc, U, c->scode, c->hoff, c->hcell, S, c->geo,   \
-                                                   c->rvec, c->vol, face, gv, dg, dth));                                \
-    else if (c->roll_g)                                                                                                \
-      LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g_direct<NEQ, T, VISC, HEAT, 1, false>), dim3(gr),             \
-                                                   dim3(kBlock), 0, c->stream, c->p, c->nc, U, c->nbr, c->mir, c->geo,  \
-                                                   c->rvec, c->vol, face, gv, dg, dth));                                \
-    else if (c->stage_u)                                                                                               \
-      LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g_direct<NEQ, T, VISC, HEAT, 4, true>), dim3(gr),              \
-                                                   dim3(kBlock), 0, c->stream, c->p, c->nc, U, c->nbr, c->mir, c->geo,  \
-                                                   c->rvec, c->vol, face, gv, dg, dth));                                \
-    else                                                                                                               \
-      LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g_direct<NEQ, T, VISC, HEAT, 4, false>), dim3(gr),             \
-                                                   dim3(kBlock), 0, c->stream, c->p, c->nc, U, c->nbr, c->mir, c->geo,  \
-                                                   c->rvec, c->vol, face, gv, dg, dth));                                \
-  } while (0)
+#define PG(NEQ, VISC, HEAT)                                                                                          \
+  LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g<NEQ, T, VISC, HEAT>), dim3(gr), dim3(kBlock), 0, c->stream,    \
+                                               c->p, c->nc, U, c->nbr, c->mir, c->geo, c->rvec, c->vol, face, gv, dg, \
+                                               dth))
   if (c->neq == 5) {
     if (heat) PG(5, true, true);
     else if (visc) PG(5, true, false);
@@ -1221,40 +984,6 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
     const int32_t bb = fc2el[2 * f + 1] >= 0 ? fc2el[2 * f + 1] : a;
     c->h_ls[(size_t)s] = 0.325 * std::pow((volume[a] + volume[bb]) / 2, 0.3333333333333);
   }
-  // pass G halo tables: per workgroup of kBlock cells (logical block index,
-  // padded to the XCD-remapped grid), the neighbour cells outside it
-  const int64_t nblk = (int64_t)grid_xcd(nElem);
-  std::vector<int32_t> hoff((size_t)nblk + 1, 0), hcell, scode((size_t)ns);
-  int halo_tab = 0;
-  {
-    std::vector<int64_t> seen((size_t)nElem, -1);
-    std::vector<int32_t> pos((size_t)nElem, 0);
-    for (int64_t b = 0; b < nblk; ++b) {
-      const int64_t c0 = b * kBlock, c1 = std::min<int64_t>(c0 + kBlock, nElem);
-      int h = 0;
-      for (int64_t cc = c0; cc < c1; ++cc)
-        for (int j = 0; j < 4; ++j) {
-          const int64_t sl = cc * 4 + j;
-          const int32_t nb = hnbr[(size_t)sl];
-          const int own = hmir[(size_t)sl] & 1;
-          if (nb < 0) {
-            scode[(size_t)sl] = nb;
-          } else if (nb >= c0 && nb < c1) {
-            scode[(size_t)sl] = (int32_t)(((nb - c0) << 1) | own);
-          } else {
-            if (seen[(size_t)nb] != b) {
-              seen[(size_t)nb] = b;
-              pos[(size_t)nb] = h++;
-              if (h <= kHaloCap) hcell.push_back(nb);
-            }
-            const int p = pos[(size_t)nb];
-            scode[(size_t)sl] = p < kHaloCap ? (int32_t)(((kBlock + p) << 1) | own) : -(8 + ((nb << 1) | own));
-          }
-        }
-      hoff[(size_t)b + 1] = (int32_t)hcell.size();
-      halo_tab = std::max(halo_tab, std::min(h, kHaloCap));
-    }
-  }
   std::vector<double> vcfl((size_t)nElem);
   for (int64_t cc = 0; cc < nElem; ++cc) vcfl[(size_t)cc] = std::pow(volume[cc], 0.3333);
   free_all(c);
@@ -1279,13 +1008,6 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
   HIPCHK(hipMalloc(&c->dscal, 16 * 8));
   HIPCHK(hipMalloc(&c->lcount, 16 * 8));
   HIPCHK(hipMemset(c->lcount, 0, 16 * 8));
-  HIPCHK(hipMalloc(&c->scode, (size_t)ns * 4));
-  HIPCHK(hipMalloc(&c->hoff, hoff.size() * 4));
-  HIPCHK(hipMalloc(&c->hcell, std::max<size_t>(1, hcell.size()) * 4));
-  HIPCHK(hipMemcpy(c->scode, scode.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->hoff, hoff.data(), hoff.size() * 4, hipMemcpyHostToDevice));
-  if (!hcell.empty()) HIPCHK(hipMemcpy(c->hcell, hcell.data(), hcell.size() * 4, hipMemcpyHostToDevice));
-  c->halo_tab = halo_tab;
   HIPCHK(hipMemcpy(c->nbr, hnbr.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->mir, hmir.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->vol, volume, cs * 8, hipMemcpyHostToDevice));
@@ -1394,18 +1116,6 @@ int fk_set_tuning(fk_ctx *c, const char *name, int64_t value) {
   }
   if (std::strcmp(name, "stage_f") == 0) {
     c->stage_f = value != 0;
-    return FK_OK;
-  }
-  if (std::strcmp(name, "stage_g") == 0) {
-    c->stage_g = value != 0;
-    return FK_OK;
-  }
-  if (std::strcmp(name, "roll_g") == 0) {
-    c->roll_g = value != 0;
-    return FK_OK;
-  }
-  if (std::strcmp(name, "stage_u") == 0) {
-    c->stage_u = value != 0;
     return FK_OK;
   }
   return fail(FK_EINVAL, std::string("unknown tuning option ") + name);

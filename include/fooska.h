@@ -158,13 +158,8 @@ int fk_set_debug(fk_ctx *ctx, int on);
  * MI355X: 8.95 + 2.44 ms vs 13.7 ms at 48M cells, mixed); 1 = the update runs
  * inside pass F.  "stage_f" (default 1): mixed precision only; pass F copies
  * the face states and velocity gradients of its workgroup's cells to LDS and
- * reads in-workgroup mirror states from there.  "stage_g" (default 0): pass G
- * converts each workgroup's cells and halo cells to primitive variables once
- * into an LDS table (else every cell converts its 4 neighbours itself).
- * "roll_g" (default 0): pass G keeps its limiter and face-state loops rolled
- * (fewer registers, less overlap).  "stage_u" (default 0): pass G copies the
- * conserved states of its workgroup's cells to LDS and reads in-workgroup
- * neighbours from there. */
+ * reads in-workgroup mirror states from there.  Unknown names return
+ * FK_EINVAL. */
 int fk_set_tuning(fk_ctx *ctx, const char *name, int64_t value);
 
 /* ---- surrogate of the relaxation Newton solve (no counterpart in the
