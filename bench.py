@@ -74,7 +74,7 @@ def measured_traffic(kernel, cells):
     return None if k is None else k["read_bytes_per_dispatch"] + k["write_bytes_per_dispatch"]
 
 
-def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None, fuse=0, stage_f=1):
+def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None, fuse=1, stage_f=1):
     """Time `steps` explicitLoop steps on the device (state resident in HBM).
     train: dict of surrogate-training options (done before the timed region)."""
     from fooskagpu_amd import GpuSolver, surrogate
@@ -147,7 +147,7 @@ def main():
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--stage-f", type=int, default=1, choices=[0, 1],
                     help="pass F stages the workgroup's face states in LDS (mixed precision); same results")
-    ap.add_argument("--fuse", type=int, default=0, choices=[0, 1],
+    ap.add_argument("--fuse", type=int, default=1, choices=[0, 1],
                     help="RK update inside pass F (1) or residual + separate update kernel (0); same results")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30,
                     help="size of the plain device copy measured as the HBM reference (0: skip)")

@@ -87,7 +87,7 @@ struct fk_ctx {
   double dt = 0, time = 0;
   int64_t step = 0;
   bool rhs_valid = false;
-  bool fuse_update = false;  // RK update inside pass F (else residual + rk_update)
+  bool fuse_update = true;   // RK update inside pass F (else residual + rk_update)
   bool stage_f = true;       // pass F stages the workgroup's face states in LDS (mixed only)
   fkx::Timer tm;
   fkx::Mlp mlp;

@@ -76,9 +76,12 @@ __device__ __forceinline__ V4<T> ld4(const void *p, int64_t i) {
 // limiter and MUSCL face states (computePrimVec.cu:38-80,
 // computeGrad.cu:51-91, computeFaceValues.cu:29-113).  Every cell converts
 // its own state and its 4 neighbours' (cons2prim, FP64) and keeps all
-// variables in registers; full face records are written once.
+// variables in registers; full face records are written once.  FP64 is held
+// to 256 VGPRs (2 waves per SIMD, no AGPR spills: 13.0 against 15.9 ms at
+// 48M cells); mixed precision runs at the compiler's choice (3 waves per
+// SIMD; 2 and 4 measured no faster).
 template <int NEQ, typename T, bool VISC, bool HEAT>
-__global__ void __launch_bounds__(kBlock) pass_g(fk_params P, int64_t nc, const double *__restrict__ U,
+__global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1)) pass_g(fk_params P, int64_t nc, const double *__restrict__ U,
                                                  const int32_t *__restrict__ nbr, const int32_t *__restrict__ mir,
                                                  const void *__restrict__ geo, const void *__restrict__ rvec,
                                                  const double *__restrict__ vol, T *__restrict__ face,
@@ -125,7 +128,15 @@ __global__ void __launch_bounds__(kBlock) pass_g(fk_params P, int64_t nc, const 
       if (!(mx[v] >= o)) mx[v] = o;
       if (!(mn[v] <= o)) mn[v] = o;
       const T avg = (W[v] + Wn[v]) / 2;
-      if (own) {
+      if constexpr (sizeof(T) == 8) {
+        // owner +, neighbour -: (-a) b = -(a b) exactly, so selecting the
+        // sign of the term equals evaluating -avg n A / V (no branch)
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+          const T t = avg * n[d] * ga.w / V;
+          g[v][d] += own ? t : -t;
+        }
+      } else if (own) {
 #pragma unroll
         for (int d = 0; d < 3; ++d) g[v][d] += avg * n[d] * ga.w / V;
       } else {
@@ -156,7 +167,11 @@ __global__ void __launch_bounds__(kBlock) pass_g(fk_params P, int64_t nc, const 
   // MUSCL face states
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
-    const V4<T> r = ld4<T>(rvec, c * 4 + j);
+    V4<T> r = ld4<T>(rvec, c * 4 + j);
+    // FP64: r g is also formed by the limiter loop; the empty asm keeps the
+    // compiler from holding those 4 NEQ products in registers until here (it
+    // recomputes them instead).  FP32 measured faster with them held.
+    if constexpr (sizeof(T) == 8) asm volatile("" : "+v"(r.x), "+v"(r.y), "+v"(r.z));
     T *out = face + (c * 4 + j) * NEQ;
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) {
@@ -246,6 +261,26 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   constexpr int NG = ngrad<NEQ, HEAT>();
   double Uc[NEQ];
   fk::load_rec<NEQ>(U + c * NEQ, Uc);
+  // fused RK stage (MODE 0): the state part of the stage, a U0 + b U, is
+  // formed here, so that the load of U0 overlaps the face loop and U need not
+  // stay live; the tail adds the residual part in the same order as
+  // rk_update (bitwise equal)
+  double base[MODE == 0 ? NEQ : 1];
+  if constexpr (MODE == 0) {
+    if (P.rkSteps == 1) {
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) base[v] = Uc[v];
+    } else {
+      double u0[NEQ];
+      fk::load_rec<NEQ>(U0 + c * NEQ, u0);
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) {
+        if (irk == 0) base[v] = u0[v];
+        else if (irk == 1) base[v] = (3.0 / 4.0) * u0[v] + (1.0 / 4.0) * Uc[v];
+        else base[v] = (1.0 / 3.0) * u0[v] + (2.0 / 3.0) * Uc[v];
+      }
+    }
+  }
   // side terms of this cell (fluxSource / computeCenterPress, two-phase)
   T avf = 0, pLiq = 0, pGas = 0;
   double pCd = 0;
@@ -421,12 +456,9 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
     for (int v = 0; v < NEQ; ++v) {
       const int neglect = (v == iz && P.neglectZ == 1);
       double u;
-      if (P.rkSteps == 1) u = (1 - neglect) * (Uc[v] + dt * (1.0 / Vc) * rhs[v]);
-      else if (irk == 0) u = (1 - neglect) * (U0[c * NEQ + v] + dt * (1.0 / Vc) * rhs[v]);
-      else if (irk == 1)
-        u = (1 - neglect) * ((3.0 / 4.0) * U0[c * NEQ + v] + (1.0 / 4.0) * Uc[v] + (1.0 / 4.0) * dt * (1.0 / Vc) * rhs[v]);
-      else
-        u = (1 - neglect) * ((1.0 / 3.0) * U0[c * NEQ + v] + (2.0 / 3.0) * Uc[v] + (2.0 / 3.0) * dt * (1.0 / Vc) * rhs[v]);
+      if (P.rkSteps == 1 || irk == 0) u = (1 - neglect) * (base[v] + dt * (1.0 / Vc) * rhs[v]);
+      else if (irk == 1) u = (1 - neglect) * (base[v] + (1.0 / 4.0) * dt * (1.0 / Vc) * rhs[v]);
+      else u = (1 - neglect) * (base[v] + (2.0 / 3.0) * dt * (1.0 / Vc) * rhs[v]);
       Unew[c * NEQ + v] = u;
     }
   }

@@ -153,10 +153,10 @@ int fk_set_clock(fk_ctx *ctx, double dt, double time, int64_t step);
 int fk_download(fk_ctx *ctx, const char *name, void *host, int64_t bytes);
 int fk_set_debug(fk_ctx *ctx, int on);
 /* Implementation choices that do not change results (no reference
- * counterpart).  "fuse_update" (default 0): 0 = pass F stores the residual
- * and a separate streaming kernel applies the RK stage update (faster on
- * MI355X: 8.95 + 2.44 ms vs 13.7 ms at 48M cells, mixed); 1 = the update runs
- * inside pass F.  "stage_f" (default 1): mixed precision only; pass F copies
+ * counterpart).  "fuse_update" (default 1): 1 = the RK stage update runs
+ * inside pass F (stages whose dt is already known; at 48M cells, mixed: 7.45 ms
+ * against 6.73 + 2.47 ms); 0 = pass F stores the residual and a separate
+ * streaming kernel applies the update.  "stage_f" (default 1): mixed precision only; pass F copies
  * the face states and velocity gradients of its workgroup's cells to LDS and
  * reads in-workgroup mirror states from there.  Unknown names return
  * FK_EINVAL. */
