@@ -129,6 +129,18 @@ __device__ __forceinline__ void cons2prim(const Mat<double> &m, const double *U,
   }
 }
 
+// FP32 primitive record of a state held in registers: the values pass G
+// would form from U itself, (float) cons2prim(u), written by the kernels that
+// produce a new state so that the next pass G gathers 4 NEQ bytes per
+// neighbour instead of 8 NEQ (mixed precision).
+template <int NEQ>
+__device__ __forceinline__ void store_prims(const Mat<double> &m, const double *u, float *__restrict__ W) {
+  double t[NEQ];
+  cons2prim<NEQ>(m, u, t);
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) W[v] = (float)t[v];
+}
+
 // ---------------- boundary ghost (DP bc.h, SP bc.h) ----------------
 template <int NEQ, typename T>
 __device__ __forceinline__ void ghost(const Mat<T> &m, int bc, const T n[3], T *s) {

@@ -89,6 +89,12 @@ struct fk_ctx {
   bool rhs_valid = false;
   bool fuse_update = true;   // RK update inside pass F (else residual + rk_update)
   bool stage_f = true;       // pass F stages the workgroup's face states in LDS (mixed only)
+  bool prims_w = true;       // mixed: producers of U also write wprim, pass G reads it
+  // FP32 primitive records [C][NEQ] of U[ia] (mixed precision), valid while
+  // w_valid: set by the kernels that write them with a new state, cleared by
+  // every other change of U
+  float *wprim = nullptr;
+  bool w_valid = false;
   fkx::Timer tm;
   fkx::Mlp mlp;
   // host geometry kept to rebuild the precision-dependent arrays

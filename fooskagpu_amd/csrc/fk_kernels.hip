@@ -80,22 +80,31 @@ __device__ __forceinline__ V4<T> ld4(const void *p, int64_t i) {
 // to 256 VGPRs (2 waves per SIMD, no AGPR spills: 13.0 against 15.9 ms at
 // 48M cells); mixed precision runs at the compiler's choice (3 waves per
 // SIMD; 2 and 4 measured no faster).
-template <int NEQ, typename T, bool VISC, bool HEAT>
-__global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1)) pass_g(fk_params P, int64_t nc, const double *__restrict__ U,
-                                                 const int32_t *__restrict__ nbr, const int32_t *__restrict__ mir,
-                                                 const void *__restrict__ geo, const void *__restrict__ rvec,
-                                                 const double *__restrict__ vol, T *__restrict__ face,
-                                                 T *__restrict__ gvel, double *__restrict__ dgrad,
-                                                 double *__restrict__ dtheta) {
+//
+// FROMW (mixed precision): the first-order states come from the FP32
+// primitive records Wp = (float) cons2prim(U) written by the producer of U
+// (same values, half the gathered bytes, no FP64 conversions here: 6.45
+// against 7.8 ms at 48M cells).
+template <int NEQ, typename T, bool VISC, bool HEAT, bool FROMW>
+__global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
+    pass_g(fk_params P, int64_t nc, const double *__restrict__ U, const float *__restrict__ Wp,
+           const int32_t *__restrict__ nbr, const int32_t *__restrict__ mir, const void *__restrict__ geo,
+           const void *__restrict__ rvec, const double *__restrict__ vol, T *__restrict__ face,
+           T *__restrict__ gvel, double *__restrict__ dgrad, double *__restrict__ dtheta) {
   const int64_t c = xcd_block() * kBlock + threadIdx.x;
   if (c >= nc) return;
   const fk::Mat<double> md = fk::make_mat<double>(P);
   const fk::Mat<T> mt = fk::make_mat<T>(P);
-  double Wd[NEQ];
-  fk::cons2prim<NEQ>(md, U + c * NEQ, Wd);
   T W[NEQ];
+  if constexpr (FROMW) {
 #pragma unroll
-  for (int v = 0; v < NEQ; ++v) W[v] = (T)Wd[v];
+    for (int v = 0; v < NEQ; ++v) W[v] = (T)Wp[c * NEQ + v];
+  } else {
+    double Wd[NEQ];
+    fk::cons2prim<NEQ>(md, U + c * NEQ, Wd);
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) W[v] = (T)Wd[v];
+  }
   T g[NEQ][3], mn[NEQ], mx[NEQ];
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) {
@@ -113,10 +122,15 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1)) pass_g(fk_pa
     const T n[3] = {ga.x, ga.y, ga.z};
     T Wn[NEQ];
     if (nb >= 0) {
-      double t[NEQ];
-      fk::cons2prim<NEQ>(md, U + (int64_t)nb * NEQ, t);
+      if constexpr (FROMW) {
 #pragma unroll
-      for (int v = 0; v < NEQ; ++v) Wn[v] = (T)t[v];
+        for (int v = 0; v < NEQ; ++v) Wn[v] = (T)Wp[(int64_t)nb * NEQ + v];
+      } else {
+        double t[NEQ];
+        fk::cons2prim<NEQ>(md, U + (int64_t)nb * NEQ, t);
+#pragma unroll
+        for (int v = 0; v < NEQ; ++v) Wn[v] = (T)t[v];
+      }
     } else {
 #pragma unroll
       for (int v = 0; v < NEQ; ++v) Wn[v] = W[v];
@@ -223,7 +237,8 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
                                                  const double *__restrict__ vol, const double *__restrict__ volcfl,
                                                  const T *__restrict__ face, const T *__restrict__ gvel,
                                                  double *__restrict__ rhs_out, double *__restrict__ dtc, int irk,
-                                                 double dt_val, const double *__restrict__ dt_ptr) {
+                                                 double dt_val, const double *__restrict__ dt_ptr,
+                                                 float *__restrict__ Wout) {
   constexpr int NGS = ngrad<NEQ, HEAT>();
   const int64_t c0 = xcd_block() * kBlock;
   const int64_t c = c0 + threadIdx.x;
@@ -452,6 +467,7 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
     const double dt = dt_ptr ? *dt_ptr : dt_val;
     const double Vc = vol[c];
     constexpr int iz = NEQ == 5 ? 3 : 5;
+    double un[NEQ];
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) {
       const int neglect = (v == iz && P.neglectZ == 1);
@@ -460,6 +476,11 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
       else if (irk == 1) u = (1 - neglect) * (base[v] + (1.0 / 4.0) * dt * (1.0 / Vc) * rhs[v]);
       else u = (1 - neglect) * (base[v] + (2.0 / 3.0) * dt * (1.0 / Vc) * rhs[v]);
       Unew[c * NEQ + v] = u;
+      un[v] = u;
+    }
+    // mixed precision: FP32 primitive record of the new state for pass G
+    if constexpr (sizeof(T) == 4) {
+      if (Wout) fk::store_prims<NEQ>(md, un, Wout + c * NEQ);
     }
   }
 }
@@ -539,7 +560,7 @@ using fk::chem_dev;
 __global__ void __launch_bounds__(kBlock) relax_therm(fk_params P, int64_t nc, double *__restrict__ U,
                                                       uint32_t *__restrict__ list,
                                                       unsigned long long *__restrict__ counts,
-                                                      unsigned long long *__restrict__ part) {
+                                                      unsigned long long *__restrict__ part, float *__restrict__ W) {
   const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   const int l = threadIdx.x & 63;
   int chem = 0, it = 0, ts = 0;
@@ -560,6 +581,7 @@ __global__ void __launch_bounds__(kBlock) relax_therm(fk_params P, int64_t nc, d
       U[c * 10 + 0] = u[0];
       U[c * 10 + 6] = u[6];
       U[c * 10 + 7] = u[7];
+      if (W) fk::store_prims<10>(fk::make_mat<double>(P), u, W + c * 10);  // chemical cells: relax_chem
     }
   }
   const unsigned long long mask = __ballot(chem);
@@ -577,7 +599,7 @@ __global__ void __launch_bounds__(kBlock) relax_therm(fk_params P, int64_t nc, d
 
 __global__ void __launch_bounds__(kBlock) relax_chem(fk_params P, double *__restrict__ U,
                                                      const uint32_t *__restrict__ list,
-                                                     unsigned long long *__restrict__ counts) {
+                                                     unsigned long long *__restrict__ counts, float *__restrict__ W) {
   const int64_t n = (int64_t)counts[5];
   unsigned long long nit = 0;
   for (int64_t e = (int64_t)blockIdx.x * kBlock + threadIdx.x; e < n; e += (int64_t)gridDim.x * kBlock) {
@@ -591,6 +613,7 @@ __global__ void __launch_bounds__(kBlock) relax_chem(fk_params P, double *__rest
     U[c * 10 + 6] = u[6];
     U[c * 10 + 7] = u[7];
     U[c * 10 + 9] = u[9];
+    if (W) fk::store_prims<10>(fk::make_mat<double>(P), u, W + c * 10);
   }
   for (int off = 32; off > 0; off >>= 1) nit += __shfl_down(nit, off, 64);
   if ((threadIdx.x & 63) == 0 && nit) atomicAdd(&counts[1], nit);
@@ -679,10 +702,12 @@ int upload_geometry(fk_ctx *c) {
 void free_all(fk_ctx *c) {
   void *ptrs[] = {c->nbr, c->mir, c->geo, c->rvec, c->wls, c->vol, c->volcfl, c->U[0], c->U[1], c->U[2], c->face,
                   c->gvel, c->rhs, c->dtc, c->red, c->dscal, c->dgrad, c->dtheta, c->lcount, c->rlist,
-                  c->rpart};
+                  c->rpart, c->wprim};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->nbr = c->mir = nullptr;
+  c->wprim = nullptr;
+  c->w_valid = false;
   c->rlist = nullptr;
   c->rpart = nullptr;
   c->rlist_cap = 0;
@@ -704,10 +729,21 @@ int launch_pass_g_t(fk_ctx *c) {
   const double *U = c->U[c->ia];
   T *face = (T *)c->face;
   T *gv = (T *)c->gvel;
-#define PG(NEQ, VISC, HEAT)                                                                                          \
-  LAUNCH_TIMED(c, "pass_g", hipLaunchKernelGGL((pass_g<NEQ, T, VISC, HEAT>), dim3(gr), dim3(kBlock), 0, c->stream,    \
-                                               c->p, c->nc, U, c->nbr, c->mir, c->geo, c->rvec, c->vol, face, gv, dg, \
-                                               dth))
+  const bool fromw = sizeof(T) == 4 && c->prims_w && c->w_valid;
+#define PG2(NEQ, VISC, HEAT, FW)                                                                                     \
+  LAUNCH_TIMED(c, "pass_g",                                                                                          \
+               hipLaunchKernelGGL((pass_g<NEQ, T, VISC, HEAT, FW>), dim3(gr), dim3(kBlock), 0, c->stream, c->p,      \
+                                  c->nc, U, c->wprim, c->nbr, c->mir, c->geo, c->rvec, c->vol, face, gv, dg, dth))
+#define PG(NEQ, VISC, HEAT)                     \
+  do {                                          \
+    if constexpr (sizeof(T) == 4) {             \
+      if (fromw) {                              \
+        PG2(NEQ, VISC, HEAT, true);             \
+        break;                                  \
+      }                                         \
+    }                                           \
+    PG2(NEQ, VISC, HEAT, false);                \
+  } while (0)
   if (c->neq == 5) {
     if (heat) PG(5, true, true);
     else if (visc) PG(5, true, false);
@@ -717,11 +753,12 @@ int launch_pass_g_t(fk_ctx *c) {
     else PG(10, false, false);
   }
 #undef PG
+#undef PG2
   return FK_OK;
 }
 
 template <typename T, int MODE>
-int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Unew) {
+int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Unew, bool write_w = false) {
   const unsigned gr = grid_xcd(c->nc);
   const bool visc = c->p.viscFlag == 1;
   const bool heat = visc && c->p.model == FK_MODEL_SINGLE && c->p.heatFlux == 1;
@@ -730,6 +767,7 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
   const T *gv = (const T *)c->gvel;
   const T *wl = (const T *)c->wls;
   const char *nm = MODE == 0 ? "pass_f" : (MODE == 1 ? "pass_f_rhs" : "pass_f_res");
+  float *Wout = (MODE == 0 && sizeof(T) == 4 && write_w) ? c->wprim : nullptr;
   // LDS staging of face states (mixed precision only: 40 KB + 9 KB per workgroup)
   const bool stage = c->stage_f && sizeof(T) == 4;
   const size_t lds = stage ? (size_t)kBlock * (4 * c->neq + 12) * sizeof(T) : 0;
@@ -738,11 +776,11 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
     if (stage)                                                                                                         \
       LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, true>), dim3(gr), dim3(kBlock), lds,     \
                                              c->stream, c->p, c->nc, U, U0, Unew, c->nbr, c->mir, c->geo, wl, c->vol,   \
-                                             c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp));                       \
+                                             c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp, Wout));                 \
     else                                                                                                               \
       LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false>), dim3(gr), dim3(kBlock), 0,      \
                                              c->stream, c->p, c->nc, U, U0, Unew, c->nbr, c->mir, c->geo, wl, c->vol,   \
-                                             c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp));                       \
+                                             c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp, Wout));                 \
   } while (0)
   if (c->neq == 5) {
     if (heat) PF(5, true, true);
@@ -771,8 +809,9 @@ int do_pass_f_rhs(fk_ctx *c) {
 int do_update(fk_ctx *c, int irk, double dt, const double *dtp);
 
 // RK stage irk from pass F: fused (MODE 0) or residual (MODE 2) + rk_update,
-// as c->fuse_update selects; both give the same bits.
-int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp) {
+// as c->fuse_update selects; both give the same bits.  want_w: the fused
+// kernel also writes the primitive records of the new state (mixed precision).
+int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want_w) {
   if (!c->fuse_update) {
     int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 2>(c, 0, 0.0, nullptr, nullptr)
                                         : launch_pass_f_t<double, 2>(c, 0, 0.0, nullptr, nullptr);
@@ -782,10 +821,12 @@ int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp) {
     return rc;
   }
   const int nb = next_buffer(c);
-  int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 0>(c, irk, dt, dtp, c->U[nb])
-                                      : launch_pass_f_t<double, 0>(c, irk, dt, dtp, c->U[nb]);
+  const bool w = want_w && c->p.precision == FK_MIXED && c->prims_w;
+  int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 0>(c, irk, dt, dtp, c->U[nb], w)
+                                      : launch_pass_f_t<double, 0>(c, irk, dt, dtp, c->U[nb], false);
   if (rc) return rc;
   c->ia = nb;
+  c->w_valid = w;
   c->rhs_valid = false;
   return FK_OK;
 }
@@ -800,6 +841,7 @@ int do_update(fk_ctx *c, int irk, double dt, const double *dtp) {
     LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update<10>), dim3(gr), dim3(kBlock), 0, c->stream, c->p, c->nc,
                                                     c->U[c->ia], c->U[c->iu0], c->U[nb], c->rhs, c->vol, irk, dt, dtp));
   c->ia = nb;
+  c->w_valid = false;
   return FK_OK;
 }
 
@@ -855,18 +897,24 @@ int do_relax(fk_ctx *c, fk_relax_stats *st, int mode = FK_RELAX_PLAIN) {
     if (!st) {  // plain relaxation that also emits training pairs, when enabled
       bool done = false;
       int rc = fk_relax_emit_impl(c, &done);
-      if (rc || done) return rc;
+      if (rc || done) {
+        c->w_valid = false;  // the emitting relaxation writes no primitive records
+        return rc;
+      }
     }
     int rc = ensure_rlist(c);
     if (rc) return rc;
     LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_ull, dim3(1), dim3(64), 0, c->stream, cnt, 6));
+    // mixed precision: the relaxation also writes the primitive records of pass G
+    float *W = (c->p.precision == FK_MIXED && c->prims_w) ? c->wprim : nullptr;
     LAUNCH_TIMED(c, "relax", hipLaunchKernelGGL(relax_therm, dim3(grid_for(c->nc)), dim3(kBlock), 0, c->stream, c->p,
-                                                c->nc, c->U[c->ia], c->rlist, cnt, c->rpart));
+                                                c->nc, c->U[c->ia], c->rlist, cnt, c->rpart, W));
     LAUNCH_TIMED(c, "relax_count", hipLaunchKernelGGL(fk::reduce_partials<3>, dim3(fk::reduce_grid(grid_for(c->nc))),
                                                       dim3(256), 0, c->stream, c->rpart, (int64_t)grid_for(c->nc), cnt));
     const unsigned gch = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_for(c->nc), 1024));
     LAUNCH_TIMED(c, "relax_chem", hipLaunchKernelGGL(relax_chem, dim3(gch), dim3(kBlock), 0, c->stream, c->p,
-                                                     c->U[c->ia], c->rlist, cnt));
+                                                     c->U[c->ia], c->rlist, cnt, W));
+    c->w_valid = W != nullptr;
   }
   if (st) {
     int rc = read_counts(c, 0, st);
@@ -944,6 +992,7 @@ int fk_set_params(fk_ctx *c, const fk_params *p) {
   const bool prec_changed = p->precision != c->p.precision;
   const bool heat_changed = (p->heatFlux != c->p.heatFlux) || (p->viscFlag != c->p.viscFlag);
   c->p = *p;
+  c->w_valid = false;
   if (c->have_mesh && (prec_changed || heat_changed)) {
     // the T-typed arrays were sized for double and for the largest NG; only refill geometry
     rc = upload_geometry(c);
@@ -1039,6 +1088,7 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
   HIPCHK(hipMalloc(&c->red, 4096 * 8));
   HIPCHK(hipMalloc(&c->dscal, 16 * 8));
   HIPCHK(hipMalloc(&c->lcount, 16 * 8));
+  HIPCHK(hipMalloc(&c->wprim, cs * neq * sizeof(float)));
   HIPCHK(hipMemset(c->lcount, 0, 16 * 8));
   HIPCHK(hipMemcpy(c->nbr, hnbr.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->mir, hmir.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
@@ -1060,6 +1110,7 @@ int fk_upload_state(fk_ctx *c, const double *U) {
   c->ia = c->iu0 = 0;
   HIPCHK(hipMemcpy(c->U[0], U, (size_t)c->nc * c->neq * 8, hipMemcpyHostToDevice));
   c->have_state = true;
+  c->w_valid = false;
   c->rhs_valid = false;
   return FK_OK;
 }
@@ -1128,7 +1179,7 @@ int fk_compute_rhs_update(fk_ctx *c, int irk, double dt) {
   if (rc) return rc;
   if (irk < 0 || irk >= c->p.rkSteps) return fail(FK_EINVAL, "bad RK stage");
   HIPCHK(hipSetDevice(c->device));
-  return do_pass_f_update(c, irk, dt, nullptr);
+  return do_pass_f_update(c, irk, dt, nullptr, true);
 }
 
 int fk_relaxation(fk_ctx *c, int mode, fk_relax_stats *st) {
@@ -1148,6 +1199,11 @@ int fk_set_tuning(fk_ctx *c, const char *name, int64_t value) {
   }
   if (std::strcmp(name, "stage_f") == 0) {
     c->stage_f = value != 0;
+    return FK_OK;
+  }
+  if (std::strcmp(name, "prims_w") == 0) {
+    c->prims_w = value != 0;
+    c->w_valid = false;
     return FK_OK;
   }
   return fail(FK_EINVAL, std::string("unknown tuning option ") + name);
@@ -1204,16 +1260,20 @@ int fk_run_steps(fk_ctx *c, int nsteps, double *dt_last, double *phys_time) {
   for (int s = 0; s < nsteps; ++s) {
     if (p.rkSteps > 1) c->iu0 = c->ia;
     const bool newdt = p.adaptDt == 1 && c->step % p.adaptDtItr == 0;
+    // the last stage's state goes to the relaxation, which writes the
+    // primitive records itself
+    const bool relax_next = p.model == FK_MODEL_TWO_PHASE && p.relaxFlag == 1;
     for (int irk = 0; irk < p.rkSteps; ++irk) {
+      const bool want_w = irk < p.rkSteps - 1 || !relax_next;
       if ((rc = do_pass_g(c))) return rc;
       if (irk == 0 && newdt) {
         if ((rc = do_pass_f_rhs(c))) return rc;
         if ((rc = do_dt_device(c))) return rc;
         if ((rc = do_update(c, irk, 0.0, c->dscal))) return rc;
       } else if (p.adaptDt == 1) {
-        if ((rc = do_pass_f_update(c, irk, 0.0, c->dscal))) return rc;
+        if ((rc = do_pass_f_update(c, irk, 0.0, c->dscal, want_w))) return rc;
       } else {
-        if ((rc = do_pass_f_update(c, irk, p.dt, nullptr))) return rc;
+        if ((rc = do_pass_f_update(c, irk, p.dt, nullptr, want_w))) return rc;
       }
     }
     if (p.adaptDt == 1)

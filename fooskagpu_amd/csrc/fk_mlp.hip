@@ -386,7 +386,7 @@ constexpr int kColdIters = 4;
 __global__ void __launch_bounds__(256) relax_screen_k(fk_params P, int64_t nc, double *__restrict__ U,
                                                       uint32_t *__restrict__ list,
                                                       unsigned long long *__restrict__ counts,
-                                                      unsigned long long *__restrict__ part) {
+                                                      unsigned long long *__restrict__ part, float *__restrict__ W) {
   const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const int l = threadIdx.x & 63;
   int chem = 0, it = 0, push = 0;
@@ -417,6 +417,9 @@ __global__ void __launch_bounds__(256) relax_screen_k(fk_params P, int64_t nc, d
         push = 1;
       }
     }
+    // primitive records of pass G for the cells solved here (list cells:
+    // relax_list_k)
+    if (W && !push) fk::store_prims<10>(fk::make_mat<double>(P), u, W + c * 10);
   }
   // wave-aggregated append to the list (its order does not affect results:
   // every cell is solved independently)
@@ -442,7 +445,7 @@ template <bool WARM>
 __global__ void __launch_bounds__(256) relax_list_k(fk_params P, double *__restrict__ U,
                                                     const uint32_t *__restrict__ list, const float *__restrict__ w,
                                                     const uint16_t *__restrict__ wbg, Norm nm,
-                                                    unsigned long long *__restrict__ counts) {
+                                                    unsigned long long *__restrict__ counts, float *__restrict__ W) {
   __shared__ __attribute__((aligned(16))) uint16_t wb[8192];
   __shared__ FwdLds lds[4];
   const int64_t n = (int64_t)counts[5];
@@ -509,6 +512,7 @@ __global__ void __launch_bounds__(256) relax_list_k(fk_params P, double *__restr
     }
     nit += it;
     for (int v = 0; v < 10; ++v) U[c * 10 + v] = uu[v];
+    if (W) fk::store_prims<10>(fk::make_mat<double>(P), uu, W + c * 10);
   }
   for (int off = 32; off > 0; off >>= 1) {
     nit += __shfl_down(nit, off, 64);
@@ -631,18 +635,21 @@ int fk_relax_mlp_impl(fk_ctx *c, bool warm) {
   unsigned long long *cnt = (unsigned long long *)c->lcount;
   LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_u64_k, dim3(1), dim3(64), 0, c->stream, cnt, 6));
   const unsigned gs = (unsigned)((c->nc + 255) / 256);
+  // mixed precision: the relaxation also writes the primitive records of pass G
+  float *W = (c->p.precision == FK_MIXED && c->prims_w) ? c->wprim : nullptr;
   LAUNCH_TIMED(c, "relax_screen", hipLaunchKernelGGL(relax_screen_k, dim3(gs), dim3(256), 0, c->stream, c->p, c->nc,
-                                                     c->U[c->ia], c->rlist, cnt, c->rpart));
+                                                     c->U[c->ia], c->rlist, cnt, c->rpart, W));
   LAUNCH_TIMED(c, "relax_count", hipLaunchKernelGGL(fk::reduce_partials<2>, dim3(fk::reduce_grid(gs)), dim3(256), 0,
                                                     c->stream, c->rpart, (int64_t)gs, cnt));
   const int64_t nchunk = (c->nc + 63) / 64;
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nchunk + 3) / 4, 1024));
   if (warm)
     LAUNCH_TIMED(c, "relax_list", hipLaunchKernelGGL(relax_list_k<true>, dim3(grid), dim3(256), 0, c->stream, c->p,
-                                                     c->U[c->ia], c->rlist, m.w, m.wb, norm_of(m), cnt));
+                                                     c->U[c->ia], c->rlist, m.w, m.wb, norm_of(m), cnt, W));
   else
     LAUNCH_TIMED(c, "relax_list", hipLaunchKernelGGL(relax_list_k<false>, dim3(grid), dim3(256), 0, c->stream, c->p,
-                                                     c->U[c->ia], c->rlist, m.w, m.wb, norm_of(m), cnt));
+                                                     c->U[c->ia], c->rlist, m.w, m.wb, norm_of(m), cnt, W));
+  c->w_valid = W != nullptr;
   return FK_OK;
 }
 

@@ -153,13 +153,21 @@ int fk_set_clock(fk_ctx *ctx, double dt, double time, int64_t step);
 int fk_download(fk_ctx *ctx, const char *name, void *host, int64_t bytes);
 int fk_set_debug(fk_ctx *ctx, int on);
 /* Implementation choices that do not change results (no reference
- * counterpart).  "fuse_update" (default 1): 1 = the RK stage update runs
- * inside pass F (stages whose dt is already known; at 48M cells, mixed: 7.45 ms
- * against 6.73 + 2.47 ms); 0 = pass F stores the residual and a separate
- * streaming kernel applies the update.  "stage_f" (default 1): mixed precision only; pass F copies
- * the face states and velocity gradients of its workgroup's cells to LDS and
- * reads in-workgroup mirror states from there.  Unknown names return
- * FK_EINVAL. */
+ * counterpart; all give bitwise equal states).
+ *  "fuse_update" (default 1): the RK stage update runs inside pass F at every
+ *     stage whose dt is known before pass F (not at a stage that takes a new
+ *     dt: there pass F stores the residual and the signal speeds, dt is
+ *     reduced, and a streaming kernel applies the update); at 48M cells,
+ *     mixed: 7.5 ms against 6.7 + 2.5 ms.  0 = always residual + update
+ *     kernel.
+ *  "stage_f" (default 1): mixed precision only; pass F copies the face states
+ *     and velocity gradients of its workgroup's cells to LDS and reads
+ *     in-workgroup mirror states from there.
+ *  "prims_w" (default 1): mixed precision only; the kernels that produce a
+ *     new state (fused pass F, relaxation) also write its FP32 primitive
+ *     variables, which the next pass G gathers instead of converting the FP64
+ *     conserved state of the cell and its neighbours.
+ * Unknown names return FK_EINVAL. */
 int fk_set_tuning(fk_ctx *ctx, const char *name, int64_t value);
 
 /* ---- surrogate of the relaxation Newton solve (no counterpart in the

@@ -176,17 +176,40 @@ def test_cons_calc_matches_oracle(gpu):
 
 @pytest.mark.parametrize("precision", [_lib.FP64, _lib.MIXED])
 def test_tuning_options_do_not_change_results(gpu, precision):
-    """fuse_update (RK update inside pass F or separate) and stage_f (LDS
-    staging of face states in pass F) are implementation choices: bitwise
-    equal states after several steps."""
+    """fuse_update (RK update inside pass F or residual + separate update),
+    stage_f (LDS staging of face states in pass F) and prims_w (pass G reads
+    FP32 primitive records written by the producers of U) are implementation
+    choices: bitwise equal states after several steps."""
     c = SMALL["shock_bubble"]()
     p = dict(c.params, precision=precision)
     out = []
-    for fuse, stage_f in ((0, 0), (1, 0), (0, 1), (1, 1)):
+    for fuse, stage_f, prims_w in ((0, 0, 0), (1, 0, 1), (0, 1, 1), (1, 1, 0), (1, 1, 1)):
         with GpuSolver(p, c.mesh, c.U) as g:
             g.set_tuning("fuse_update", fuse)
             g.set_tuning("stage_f", stage_f)
+            g.set_tuning("prims_w", prims_w)
             g.run_steps(3)
             out.append(g.download_state())
     for o in out[1:]:
         assert np.array_equal(o, out[0])
+
+
+@pytest.mark.parametrize("name,mode", [("cavitation", _lib.RELAX_PLAIN), ("cavitation", _lib.RELAX_COLD),
+                                       ("shock_bubble", _lib.RELAX_COLD)])
+def test_primitive_records_of_relaxation(gpu, name, mode):
+    """Mixed precision: the relaxation kernels (plain: thermal + chemical
+    list; surrogate screen + list) write the FP32 primitive records that the
+    next pass G reads; the run is bitwise that of pass G converting U itself."""
+    c = SMALL[name]()
+    p = dict(c.params, precision=_lib.MIXED, relaxMode=mode)
+    steps = STEPS.get(name, 3)
+    out = []
+    for prims_w in (0, 1):
+        with GpuSolver(p, c.mesh, c.U) as g:
+            g.set_tuning("prims_w", prims_w)
+            g.run_steps(steps)
+            out.append(g.download_state())
+            st = g.relax_totals()
+    print(name, mode, st)
+    assert np.isfinite(out[0]).all()
+    assert np.array_equal(out[1], out[0])
