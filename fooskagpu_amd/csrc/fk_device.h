@@ -102,6 +102,19 @@ __device__ __forceinline__ void load_rec(const double *__restrict__ p, double (&
   }
 }
 
+// Stores N doubles of one AoS record with 16-byte stores when N is even.
+template <int N>
+__device__ __forceinline__ void store_rec(double *__restrict__ p, const double (&u)[N]) {
+  if constexpr (N % 2 == 0) {
+    double2 *q = reinterpret_cast<double2 *>(p);
+#pragma unroll
+    for (int i = 0; i < N / 2; ++i) q[i] = make_double2(u[2 * i], u[2 * i + 1]);
+  } else {
+#pragma unroll
+    for (int i = 0; i < N; ++i) p[i] = u[i];
+  }
+}
+
 // two-phase: DP cons2prim.h:7-22; single-phase: SP cons2prim.h:2-10
 template <int NEQ>
 __device__ __forceinline__ void cons2prim(const Mat<double> &m, const double *U, double *s) {

@@ -507,6 +507,38 @@ __global__ void __launch_bounds__(kBlock) rk_update(fk_params P, int64_t nc, con
   Unew[i] = u;
 }
 
+// The same RK stage one cell per thread (16-byte record loads and stores),
+// also writing the FP32 primitive record of the new state for pass G (mixed
+// precision).  Same expressions as rk_update, so the same bits.
+template <int NEQ>
+__global__ void __launch_bounds__(kBlock) rk_update_w(fk_params P, int64_t nc, const double *__restrict__ U,
+                                                      const double *__restrict__ U0, double *__restrict__ Unew,
+                                                      const double *__restrict__ rhs, const double *__restrict__ vol,
+                                                      int irk, double dt_val, const double *__restrict__ dt_ptr,
+                                                      float *__restrict__ W) {
+  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
+  if (c >= nc) return;
+  const double dt = dt_ptr ? *dt_ptr : dt_val;
+  const double Vc = vol[c];
+  constexpr int iz = NEQ == 5 ? 3 : 5;
+  double r[NEQ], uc[NEQ], u0[NEQ], un[NEQ];
+  fk::load_rec<NEQ>(rhs + c * NEQ, r);
+  if (P.rkSteps == 1 || irk != 0) fk::load_rec<NEQ>(U + c * NEQ, uc);
+  if (P.rkSteps != 1) fk::load_rec<NEQ>(U0 + c * NEQ, u0);
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) {
+    const int neglect = (v == iz && P.neglectZ == 1);
+    double u;
+    if (P.rkSteps == 1) u = (1 - neglect) * (uc[v] + dt * (1.0 / Vc) * r[v]);
+    else if (irk == 0) u = (1 - neglect) * (u0[v] + dt * (1.0 / Vc) * r[v]);
+    else if (irk == 1) u = (1 - neglect) * ((3.0 / 4.0) * u0[v] + (1.0 / 4.0) * uc[v] + (1.0 / 4.0) * dt * (1.0 / Vc) * r[v]);
+    else u = (1 - neglect) * ((1.0 / 3.0) * u0[v] + (2.0 / 3.0) * uc[v] + (2.0 / 3.0) * dt * (1.0 / Vc) * r[v]);
+    un[v] = u;
+  }
+  fk::store_rec<NEQ>(Unew + c * NEQ, un);
+  fk::store_prims<NEQ>(fk::make_mat<double>(P), un, W + c * NEQ);
+}
+
 // min over cells of Vcfl/signal: per-block minima, then one block folds them
 // and forms dt = min(cfl * min, maxDt) on the device.
 __global__ void __launch_bounds__(kBlock) min_blocks(int64_t n, const double *__restrict__ x, double *__restrict__ out) {
@@ -806,7 +838,7 @@ int do_pass_f_rhs(fk_ctx *c) {
   return FK_OK;
 }
 
-int do_update(fk_ctx *c, int irk, double dt, const double *dtp);
+int do_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want_w);
 
 // RK stage irk from pass F: fused (MODE 0) or residual (MODE 2) + rk_update,
 // as c->fuse_update selects; both give the same bits.  want_w: the fused
@@ -816,7 +848,7 @@ int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want
     int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 2>(c, 0, 0.0, nullptr, nullptr)
                                         : launch_pass_f_t<double, 2>(c, 0, 0.0, nullptr, nullptr);
     if (rc) return rc;
-    rc = do_update(c, irk, dt, dtp);
+    rc = do_update(c, irk, dt, dtp, want_w);
     c->rhs_valid = false;
     return rc;
   }
@@ -831,8 +863,22 @@ int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want
   return FK_OK;
 }
 
-int do_update(fk_ctx *c, int irk, double dt, const double *dtp) {
+int do_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want_w) {
   const int nb = next_buffer(c);
+  if (want_w && c->p.precision == FK_MIXED && c->prims_w) {
+    const unsigned gc = grid_for(c->nc);
+    if (c->neq == 5)
+      LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update_w<5>), dim3(gc), dim3(kBlock), 0, c->stream, c->p,
+                                                      c->nc, c->U[c->ia], c->U[c->iu0], c->U[nb], c->rhs, c->vol, irk,
+                                                      dt, dtp, c->wprim));
+    else
+      LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update_w<10>), dim3(gc), dim3(kBlock), 0, c->stream, c->p,
+                                                      c->nc, c->U[c->ia], c->U[c->iu0], c->U[nb], c->rhs, c->vol, irk,
+                                                      dt, dtp, c->wprim));
+    c->ia = nb;
+    c->w_valid = true;
+    return FK_OK;
+  }
   const unsigned gr = grid_for(c->nc * c->neq);
   if (c->neq == 5)
     LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update<5>), dim3(gr), dim3(kBlock), 0, c->stream, c->p, c->nc,
@@ -1169,7 +1215,7 @@ int fk_update(fk_ctx *c, int irk, double dt) {
   if (!c->rhs_valid) return fail(FK_ESTATE, "fk_compute_rhs must run before fk_update");
   if (irk < 0 || irk >= c->p.rkSteps) return fail(FK_EINVAL, "bad RK stage");
   HIPCHK(hipSetDevice(c->device));
-  rc = do_update(c, irk, dt, nullptr);
+  rc = do_update(c, irk, dt, nullptr, true);
   c->rhs_valid = false;
   return rc;
 }
@@ -1269,7 +1315,7 @@ int fk_run_steps(fk_ctx *c, int nsteps, double *dt_last, double *phys_time) {
       if (irk == 0 && newdt) {
         if ((rc = do_pass_f_rhs(c))) return rc;
         if ((rc = do_dt_device(c))) return rc;
-        if ((rc = do_update(c, irk, 0.0, c->dscal))) return rc;
+        if ((rc = do_update(c, irk, 0.0, c->dscal, want_w))) return rc;
       } else if (p.adaptDt == 1) {
         if ((rc = do_pass_f_update(c, irk, 0.0, c->dscal, want_w))) return rc;
       } else {
