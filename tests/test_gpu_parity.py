@@ -174,13 +174,15 @@ def test_cons_calc_matches_oracle(gpu):
     assert np.allclose(a, b, rtol=1e-12, atol=0)
 
 
+@pytest.mark.parametrize("name", ["shock_bubble", "tgv", "sod"])
 @pytest.mark.parametrize("precision", [_lib.FP64, _lib.MIXED])
-def test_tuning_options_do_not_change_results(gpu, precision):
+def test_tuning_options_do_not_change_results(gpu, precision, name):
     """fuse_update (RK update inside pass F or residual + separate update),
     stage_f (LDS staging of face states in pass F) and prims_w (pass G reads
     FP32 primitive records written by the producers of U) are implementation
-    choices: bitwise equal states after several steps."""
-    c = SMALL["shock_bubble"]()
+    choices: bitwise equal states after several steps (two-phase with
+    relaxation, single-phase viscous, single-phase inviscid)."""
+    c = SMALL[name]()
     p = dict(c.params, precision=precision)
     out = []
     for fuse, stage_f, prims_w in ((0, 0, 0), (1, 0, 1), (0, 1, 1), (1, 1, 0), (1, 1, 1)):
