@@ -64,7 +64,7 @@ struct fk_ctx {
   int64_t nc = 0, nf = 0;
   bool have_mesh = false, have_state = false, debug = false, timing = false;
   // mesh
-  int32_t *nbr = nullptr, *mir = nullptr;
+  int32_t *link = nullptr;  // [4C] slot links (see fk_kernels.hip)
   void *geo = nullptr, *rvec = nullptr, *wls = nullptr;  // T-typed, sized for double
   double *vol = nullptr, *volcfl = nullptr;
   std::vector<int32_t> slot_face;  // host copy of slot -> face id

@@ -4,10 +4,13 @@
 // Layout in HBM (one context):
 //   U        3 rotating buffers [nElem][NEQ] double (reference AoS consVec)
 //   slots    4 per cell, ordered by ascending face id:
-//              nbr[4C]   neighbour cell, or -bc on boundary faces
-//              mir[4C]   (mirror slot in the neighbour << 1) | owner flag
+//              link[4C]  (mirror slot in the neighbour << 1) | owner flag,
+//                        or -bc on boundary faces (owner side); the
+//                        neighbour cell is mirror slot / 4
 //              geo[4C]   (n.x, n.y, n.z, A) of the face (owner normal)
-//              rvec[4C]  (x_f - x_c) of the face seen from this cell
+//              rvec[4C]  (x_f - x_c) of the face seen from this cell, and the
+//                        slot link in the 4th word (bits of an int32 in FP32,
+//                        its value in FP64) so pass G reads no link array
 //              wls[4C]   WALE filter width 0.325 ((V_o+V_n)/2)^0.3333333333333
 //   face     [4C][NEQ] T: MUSCL face state of every cell face slot
 //   gvel     [C][NG]  T: velocity gradient (+ temperature gradient)
@@ -72,6 +75,15 @@ __device__ __forceinline__ V4<T> ld4(const void *p, int64_t i) {
   return reinterpret_cast<const V4<T> *>(p)[i];
 }
 
+// slot link -> neighbour cell (or -bc) and owner flag
+__device__ __forceinline__ int link_nb(int lk) { return lk >= 0 ? (lk >> 3) : lk; }
+__device__ __forceinline__ bool link_own(int lk) { return lk < 0 || (lk & 1); }
+template <typename T>
+__device__ __forceinline__ int link_of(T w) {
+  if constexpr (sizeof(T) == 4) return __float_as_int(w);
+  else return (int)w;
+}
+
 // Pass G: first-order states, Green-Gauss gradient, neighbour min/max,
 // limiter and MUSCL face states (computePrimVec.cu:38-80,
 // computeGrad.cu:51-91, computeFaceValues.cu:29-113).  Every cell converts
@@ -88,8 +100,8 @@ __device__ __forceinline__ V4<T> ld4(const void *p, int64_t i) {
 template <int NEQ, typename T, bool VISC, bool HEAT, bool FROMW>
 __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
     pass_g(fk_params P, int64_t nc, const double *__restrict__ U, const float *__restrict__ Wp,
-           const int32_t *__restrict__ nbr, const int32_t *__restrict__ mir, const void *__restrict__ geo,
-           const void *__restrict__ rvec, const double *__restrict__ vol, T *__restrict__ face,
+           const int32_t *__restrict__ link, const void *__restrict__ geo, const void *__restrict__ rvec,
+           const double *__restrict__ vol, T *__restrict__ face,
            T *__restrict__ gvel, double *__restrict__ dgrad, double *__restrict__ dtheta) {
   const int64_t c = xcd_block() * kBlock + threadIdx.x;
   if (c >= nc) return;
@@ -116,8 +128,13 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
 #pragma unroll 1
   for (int j = 0; j < 4; ++j) {
     const int64_t s = c * 4 + j;
-    const int nb = nbr[s];
-    const bool own = mir[s] & 1;
+    // mixed: the link rides in rvec's 4th word (one 16-byte load per slot);
+    // FP64 (32-byte rvec records) measured faster with the 4-byte link array
+    int lk;
+    if constexpr (sizeof(T) == 4) lk = link_of<T>(ld4<T>(rvec, s).w);
+    else lk = link[s];
+    const int nb = link_nb(lk);
+    const bool own = link_own(lk);
     const V4<T> ga = ld4<T>(geo, s);
     const T n[3] = {ga.x, ga.y, ga.z};
     T Wn[NEQ];
@@ -232,7 +249,7 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
 template <int NEQ, typename T, bool VISC, bool HEAT, int MODE, bool STAGE>
 __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3)) pass_f(fk_params P, int64_t nc, const double *__restrict__ U,
                                                  const double *__restrict__ U0, double *__restrict__ Unew,
-                                                 const int32_t *__restrict__ nbr, const int32_t *__restrict__ mir,
+                                                 const int32_t *__restrict__ link,
                                                  const void *__restrict__ geo, const T *__restrict__ wls,
                                                  const double *__restrict__ vol, const double *__restrict__ volcfl,
                                                  const T *__restrict__ face, const T *__restrict__ gvel,
@@ -325,9 +342,9 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   }
   for (int j = 0; j < 4; ++j) {
     const int64_t s = c * 4 + j;
-    const int nb = nbr[s];
-    const int mv = mir[s];
-    const bool own = mv & 1;
+    const int mv = link[s];
+    const int nb = link_nb(mv);
+    const bool own = link_own(mv);
     const V4<T> ga = ld4<T>(geo, s);
     const T n[3] = {ga.x, ga.y, ga.z};
     const T A = ga.w;
@@ -718,7 +735,14 @@ int upload_geometry(fk_ctx *c) {
   if (c->p.precision == FK_MIXED) {
     std::vector<float> g((size_t)ns * 4), r((size_t)ns * 4), l((size_t)ns);
     for (size_t i = 0; i < g.size(); ++i) g[i] = (float)c->h_geo[i];
-    for (size_t i = 0; i < r.size(); ++i) r[i] = (float)c->h_r[i];
+    for (size_t i = 0; i < r.size(); ++i) {
+      if (i % 4 == 3) {  // the slot link, kept bit-exact
+        const int32_t lk = (int32_t)c->h_r[i];
+        std::memcpy(&r[i], &lk, 4);
+      } else {
+        r[i] = (float)c->h_r[i];
+      }
+    }
     for (size_t i = 0; i < l.size(); ++i) l[i] = (float)c->h_ls[i];
     HIPCHK(hipMemcpy(c->geo, g.data(), g.size() * 4, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(c->rvec, r.data(), r.size() * 4, hipMemcpyHostToDevice));
@@ -732,12 +756,12 @@ int upload_geometry(fk_ctx *c) {
 }
 
 void free_all(fk_ctx *c) {
-  void *ptrs[] = {c->nbr, c->mir, c->geo, c->rvec, c->wls, c->vol, c->volcfl, c->U[0], c->U[1], c->U[2], c->face,
+  void *ptrs[] = {c->link, c->geo, c->rvec, c->wls, c->vol, c->volcfl, c->U[0], c->U[1], c->U[2], c->face,
                   c->gvel, c->rhs, c->dtc, c->red, c->dscal, c->dgrad, c->dtheta, c->lcount, c->rlist,
                   c->rpart, c->wprim};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
-  c->nbr = c->mir = nullptr;
+  c->link = nullptr;
   c->wprim = nullptr;
   c->w_valid = false;
   c->rlist = nullptr;
@@ -765,7 +789,7 @@ int launch_pass_g_t(fk_ctx *c) {
 #define PG2(NEQ, VISC, HEAT, FW)                                                                                     \
   LAUNCH_TIMED(c, "pass_g",                                                                                          \
                hipLaunchKernelGGL((pass_g<NEQ, T, VISC, HEAT, FW>), dim3(gr), dim3(kBlock), 0, c->stream, c->p,      \
-                                  c->nc, U, c->wprim, c->nbr, c->mir, c->geo, c->rvec, c->vol, face, gv, dg, dth))
+                                  c->nc, U, c->wprim, c->link, c->geo, c->rvec, c->vol, face, gv, dg, dth))
 #define PG(NEQ, VISC, HEAT)                     \
   do {                                          \
     if constexpr (sizeof(T) == 4) {             \
@@ -807,11 +831,11 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
   do {                                                                                                                 \
     if (stage)                                                                                                         \
       LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, true>), dim3(gr), dim3(kBlock), lds,     \
-                                             c->stream, c->p, c->nc, U, U0, Unew, c->nbr, c->mir, c->geo, wl, c->vol,   \
+                                             c->stream, c->p, c->nc, U, U0, Unew, c->link, c->geo, wl, c->vol,          \
                                              c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp, Wout));                 \
     else                                                                                                               \
       LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false>), dim3(gr), dim3(kBlock), 0,      \
-                                             c->stream, c->p, c->nc, U, U0, Unew, c->nbr, c->mir, c->geo, wl, c->vol,   \
+                                             c->stream, c->p, c->nc, U, U0, Unew, c->link, c->geo, wl, c->vol,          \
                                              c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp, Wout));                 \
   } while (0)
   if (c->neq == 5) {
@@ -1076,7 +1100,7 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
   for (int64_t cc = 0; cc < nElem; ++cc)
     if (cnt[(size_t)cc] != 4) return fail(FK_EINVAL, "every cell must have exactly four faces (tetrahedra)");
   const int64_t ns = nElem * 4;
-  std::vector<int32_t> hnbr((size_t)ns), hmir((size_t)ns);
+  std::vector<int32_t> hlink((size_t)ns);
   c->h_geo.assign((size_t)ns * 4, 0.0);
   c->h_r.assign((size_t)ns * 4, 0.0);
   c->h_ls.assign((size_t)ns, 0.0);
@@ -1094,12 +1118,12 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
     const int side = fc2el[2 * f] == cc ? 0 : 1;
     const int32_t other = fc2el[2 * f + (1 - side)];
     if (other >= 0) {
-      hnbr[(size_t)s] = other;
-      hmir[(size_t)s] = (fslot[(size_t)(2 * f + 1 - side)] << 1) | (side == 0 ? 1 : 0);
+      hlink[(size_t)s] = (fslot[(size_t)(2 * f + 1 - side)] << 1) | (side == 0 ? 1 : 0);
     } else {
-      hnbr[(size_t)s] = -boundCond[f];
-      hmir[(size_t)s] = 1;
+      if (boundCond[f] < 1) return fail(FK_EINVAL, "boundary face without a boundary condition (boundCond < 1)");
+      hlink[(size_t)s] = -boundCond[f];
     }
+    c->h_r[(size_t)(s * 4 + 3)] = (double)hlink[(size_t)s];
     for (int d = 0; d < 3; ++d) c->h_geo[(size_t)(s * 4 + d)] = n[3 * f + d];
     c->h_geo[(size_t)(s * 4 + 3)] = area[f];
     for (int d = 0; d < 3; ++d) {
@@ -1119,8 +1143,7 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
   c->nf = nFcs;
   const int neq = c->neq;
   const size_t cs = (size_t)nElem;
-  HIPCHK(hipMalloc(&c->nbr, (size_t)ns * 4));
-  HIPCHK(hipMalloc(&c->mir, (size_t)ns * 4));
+  HIPCHK(hipMalloc(&c->link, (size_t)ns * 4));
   HIPCHK(hipMalloc(&c->geo, (size_t)ns * 4 * 8));
   HIPCHK(hipMalloc(&c->rvec, (size_t)ns * 4 * 8));
   HIPCHK(hipMalloc(&c->wls, (size_t)ns * 8));
@@ -1136,8 +1159,7 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
   HIPCHK(hipMalloc(&c->lcount, 16 * 8));
   HIPCHK(hipMalloc(&c->wprim, cs * neq * sizeof(float)));
   HIPCHK(hipMemset(c->lcount, 0, 16 * 8));
-  HIPCHK(hipMemcpy(c->nbr, hnbr.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
-  HIPCHK(hipMemcpy(c->mir, hmir.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c->link, hlink.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->vol, volume, cs * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(c->volcfl, vcfl.data(), cs * 8, hipMemcpyHostToDevice));
   HIPCHK(hipMemset(c->dscal, 0, 16 * 8));
