@@ -246,7 +246,12 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
 // each) and their velocity gradients into LDS; the mirror states and
 // neighbour gradients of faces inside the workgroup are then read from LDS,
 // only those of faces to other workgroups from global memory.
-template <int NEQ, typename T, bool VISC, bool HEAT, int MODE, bool STAGE>
+//
+// WOUT (MODE 0, mixed): also write the FP32 primitive record of the new state.
+// A template flag rather than a null test: the FP64 conversion at the end
+// makes the kernel spill at 3 waves per SIMD (8.4 against 7.6 ms), so the
+// stages whose records are not read use the variant without it.
+template <int NEQ, typename T, bool VISC, bool HEAT, int MODE, bool STAGE, bool WOUT = false>
 __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3)) pass_f(fk_params P, int64_t nc, const double *__restrict__ U,
                                                  const double *__restrict__ U0, double *__restrict__ Unew,
                                                  const int32_t *__restrict__ link,
@@ -496,9 +501,7 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
       un[v] = u;
     }
     // mixed precision: FP32 primitive record of the new state for pass G
-    if constexpr (sizeof(T) == 4) {
-      if (Wout) fk::store_prims<NEQ>(md, un, Wout + c * NEQ);
-    }
+    if constexpr (WOUT) fk::store_prims<NEQ>(md, un, Wout + c * NEQ);
   }
 }
 
@@ -829,6 +832,20 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
   const size_t lds = stage ? (size_t)kBlock * (4 * c->neq + 12) * sizeof(T) : 0;
 #define PF(NEQ, VISC, HEAT)                                                                                            \
   do {                                                                                                                 \
+    if constexpr (MODE == 0 && sizeof(T) == 4) {                                                                       \
+      if (Wout) {                                                                                                      \
+        if (stage)                                                                                                     \
+          LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, true, true>), dim3(gr),             \
+                                                 dim3(kBlock), lds, c->stream, c->p, c->nc, U, U0, Unew, c->link,       \
+                                                 c->geo, wl, c->vol, c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp, \
+                                                 Wout));                                                                \
+        else                                                                                                           \
+          LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false, true>), dim3(gr),            \
+                                                 dim3(kBlock), 0, c->stream, c->p, c->nc, U, U0, Unew, c->link, c->geo, \
+                                                 wl, c->vol, c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp, Wout)); \
+        break;                                                                                                         \
+      }                                                                                                                \
+    }                                                                                                                  \
     if (stage)                                                                                                         \
       LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, true>), dim3(gr), dim3(kBlock), lds,     \
                                              c->stream, c->p, c->nc, U, U0, Unew, c->link, c->geo, wl, c->vol,          \
