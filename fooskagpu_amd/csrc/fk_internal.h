@@ -87,6 +87,7 @@ struct fk_ctx {
   double dt = 0, time = 0;
   int64_t step = 0;
   bool rhs_valid = false;
+  bool rhs_f32 = false;      // c->rhs holds FP32 values (mixed dt stage, pass F MODE 1)
   bool fuse_update = true;   // RK update inside pass F (else residual + rk_update)
   bool stage_f = true;       // pass F stages the workgroup's face states in LDS (mixed only)
   bool prims_w = true;       // mixed: producers of U also write wprim, pass G reads it

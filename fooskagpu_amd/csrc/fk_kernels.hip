@@ -483,7 +483,13 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   }
   if constexpr (MODE != 0) {
 #pragma unroll
-    for (int v = 0; v < NEQ; ++v) rhs_out[c * NEQ + v] = rhs[v];
+    for (int v = 0; v < NEQ; ++v) {
+      // MODE 1 (the dt stage), mixed precision: the stored residual is FP32,
+      // promoted in the update (the MP policy); MODE 2 stores FP64 so that
+      // the split stages equal the fused ones bit for bit
+      if constexpr (MODE == 1 && sizeof(T) == 4) reinterpret_cast<float *>(rhs_out)[c * NEQ + v] = (float)rhs[v];
+      else rhs_out[c * NEQ + v] = rhs[v];
+    }
     if constexpr (MODE == 1) dtc[c] = volcfl[c] / (double)sig;
   } else {
     const double dt = dt_ptr ? *dt_ptr : dt_val;
@@ -505,14 +511,15 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   }
 }
 
-// RK stage from a stored residual (update.cu:19-103).
-template <int NEQ>
+// RK stage from a stored residual (update.cu:19-103); RT: its storage type.
+template <int NEQ, typename RT>
 __global__ void __launch_bounds__(kBlock) rk_update(fk_params P, int64_t nc, const double *__restrict__ U,
                                                     const double *__restrict__ U0, double *__restrict__ Unew,
-                                                    const double *__restrict__ rhs, const double *__restrict__ vol,
+                                                    const RT *__restrict__ rhs_s, const double *__restrict__ vol,
                                                     int irk, double dt_val, const double *__restrict__ dt_ptr) {
   const int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x;
   if (i >= nc * NEQ) return;
+  const double ri = (double)rhs_s[i];
   const int64_t c = i / NEQ;
   const int v = (int)(i - c * NEQ);
   const double dt = dt_ptr ? *dt_ptr : dt_val;
@@ -520,20 +527,20 @@ __global__ void __launch_bounds__(kBlock) rk_update(fk_params P, int64_t nc, con
   constexpr int iz = NEQ == 5 ? 3 : 5;
   const int neglect = (v == iz && P.neglectZ == 1);
   double u;
-  if (P.rkSteps == 1) u = (1 - neglect) * (U[i] + dt * (1.0 / Vc) * rhs[i]);
-  else if (irk == 0) u = (1 - neglect) * (U0[i] + dt * (1.0 / Vc) * rhs[i]);
-  else if (irk == 1) u = (1 - neglect) * ((3.0 / 4.0) * U0[i] + (1.0 / 4.0) * U[i] + (1.0 / 4.0) * dt * (1.0 / Vc) * rhs[i]);
-  else u = (1 - neglect) * ((1.0 / 3.0) * U0[i] + (2.0 / 3.0) * U[i] + (2.0 / 3.0) * dt * (1.0 / Vc) * rhs[i]);
+  if (P.rkSteps == 1) u = (1 - neglect) * (U[i] + dt * (1.0 / Vc) * ri);
+  else if (irk == 0) u = (1 - neglect) * (U0[i] + dt * (1.0 / Vc) * ri);
+  else if (irk == 1) u = (1 - neglect) * ((3.0 / 4.0) * U0[i] + (1.0 / 4.0) * U[i] + (1.0 / 4.0) * dt * (1.0 / Vc) * ri);
+  else u = (1 - neglect) * ((1.0 / 3.0) * U0[i] + (2.0 / 3.0) * U[i] + (2.0 / 3.0) * dt * (1.0 / Vc) * ri);
   Unew[i] = u;
 }
 
 // The same RK stage one cell per thread (16-byte record loads and stores),
 // also writing the FP32 primitive record of the new state for pass G (mixed
 // precision).  Same expressions as rk_update, so the same bits.
-template <int NEQ>
+template <int NEQ, typename RT>
 __global__ void __launch_bounds__(kBlock) rk_update_w(fk_params P, int64_t nc, const double *__restrict__ U,
                                                       const double *__restrict__ U0, double *__restrict__ Unew,
-                                                      const double *__restrict__ rhs, const double *__restrict__ vol,
+                                                      const RT *__restrict__ rhs, const double *__restrict__ vol,
                                                       int irk, double dt_val, const double *__restrict__ dt_ptr,
                                                       float *__restrict__ W) {
   const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
@@ -542,7 +549,12 @@ __global__ void __launch_bounds__(kBlock) rk_update_w(fk_params P, int64_t nc, c
   const double Vc = vol[c];
   constexpr int iz = NEQ == 5 ? 3 : 5;
   double r[NEQ], uc[NEQ], u0[NEQ], un[NEQ];
-  fk::load_rec<NEQ>(rhs + c * NEQ, r);
+  if constexpr (sizeof(RT) == 8) {
+    fk::load_rec<NEQ>(rhs + c * NEQ, r);
+  } else {
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) r[v] = (double)rhs[c * NEQ + v];
+  }
   if (P.rkSteps == 1 || irk != 0) fk::load_rec<NEQ>(U + c * NEQ, uc);
   if (P.rkSteps != 1) fk::load_rec<NEQ>(U0 + c * NEQ, u0);
 #pragma unroll
@@ -875,6 +887,7 @@ int do_pass_f_rhs(fk_ctx *c) {
   int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 1>(c, 0, 0.0, nullptr, nullptr)
                                       : launch_pass_f_t<double, 1>(c, 0, 0.0, nullptr, nullptr);
   if (rc) return rc;
+  c->rhs_f32 = c->p.precision == FK_MIXED;
   c->rhs_valid = true;
   return FK_OK;
 }
@@ -889,6 +902,7 @@ int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want
     int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 2>(c, 0, 0.0, nullptr, nullptr)
                                         : launch_pass_f_t<double, 2>(c, 0, 0.0, nullptr, nullptr);
     if (rc) return rc;
+    c->rhs_f32 = false;
     rc = do_update(c, irk, dt, dtp, want_w);
     c->rhs_valid = false;
     return rc;
@@ -904,31 +918,33 @@ int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want
   return FK_OK;
 }
 
+// RK stage from the stored residual (FP32 after the mixed dt stage, c->rhs_f32)
+template <int NEQ, typename RT>
+int launch_update_t(fk_ctx *c, int irk, double dt, const double *dtp, int nb, bool w) {
+  const RT *r = (const RT *)c->rhs;
+  if (w)
+    LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update_w<NEQ, RT>), dim3(grid_for(c->nc)), dim3(kBlock), 0,
+                                                    c->stream, c->p, c->nc, c->U[c->ia], c->U[c->iu0], c->U[nb], r,
+                                                    c->vol, irk, dt, dtp, c->wprim));
+  else
+    LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update<NEQ, RT>), dim3(grid_for(c->nc * NEQ)), dim3(kBlock), 0,
+                                                    c->stream, c->p, c->nc, c->U[c->ia], c->U[c->iu0], c->U[nb], r,
+                                                    c->vol, irk, dt, dtp));
+  return FK_OK;
+}
+
 int do_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want_w) {
   const int nb = next_buffer(c);
-  if (want_w && c->p.precision == FK_MIXED && c->prims_w) {
-    const unsigned gc = grid_for(c->nc);
-    if (c->neq == 5)
-      LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update_w<5>), dim3(gc), dim3(kBlock), 0, c->stream, c->p,
-                                                      c->nc, c->U[c->ia], c->U[c->iu0], c->U[nb], c->rhs, c->vol, irk,
-                                                      dt, dtp, c->wprim));
-    else
-      LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update_w<10>), dim3(gc), dim3(kBlock), 0, c->stream, c->p,
-                                                      c->nc, c->U[c->ia], c->U[c->iu0], c->U[nb], c->rhs, c->vol, irk,
-                                                      dt, dtp, c->wprim));
-    c->ia = nb;
-    c->w_valid = true;
-    return FK_OK;
-  }
-  const unsigned gr = grid_for(c->nc * c->neq);
+  const bool w = want_w && c->p.precision == FK_MIXED && c->prims_w;
+  int rc;
   if (c->neq == 5)
-    LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update<5>), dim3(gr), dim3(kBlock), 0, c->stream, c->p, c->nc,
-                                                    c->U[c->ia], c->U[c->iu0], c->U[nb], c->rhs, c->vol, irk, dt, dtp));
+    rc = c->rhs_f32 ? launch_update_t<5, float>(c, irk, dt, dtp, nb, w) : launch_update_t<5, double>(c, irk, dt, dtp, nb, w);
   else
-    LAUNCH_TIMED(c, "rk_update", hipLaunchKernelGGL((rk_update<10>), dim3(gr), dim3(kBlock), 0, c->stream, c->p, c->nc,
-                                                    c->U[c->ia], c->U[c->iu0], c->U[nb], c->rhs, c->vol, irk, dt, dtp));
+    rc = c->rhs_f32 ? launch_update_t<10, float>(c, irk, dt, dtp, nb, w)
+                    : launch_update_t<10, double>(c, irk, dt, dtp, nb, w);
+  if (rc) return rc;
   c->ia = nb;
-  c->w_valid = false;
+  c->w_valid = w;
   return FK_OK;
 }
 
@@ -1436,7 +1452,16 @@ int fk_download(fk_ctx *c, const char *name, void *host, int64_t bytes) {
   if (s == "grad") return get(c->dgrad, nc * neq * 3 * 8);
   if (s == "theta") return get(c->dtheta, nc * neq * 8);
   if (s == "face") return get(c->face, nc * 4 * neq * (int64_t)ts);
-  if (s == "rhs") return get(c->rhs, nc * neq * 8);
+  if (s == "rhs") {
+    if (!c->rhs_f32) return get(c->rhs, nc * neq * 8);
+    // stored as FP32 by the mixed dt stage: widen on the host
+    if (bytes != nc * neq * 8) return fail(FK_EINVAL, "size mismatch for rhs");
+    std::vector<float> f((size_t)(nc * neq));
+    HIPCHK(hipMemcpy(f.data(), c->rhs, f.size() * 4, hipMemcpyDeviceToHost));
+    double *o = (double *)host;
+    for (size_t i = 0; i < f.size(); ++i) o[i] = f[i];
+    return FK_OK;
+  }
   if (s == "signal") {
     // dtc holds Vcfl / signal; return the signal itself
     std::vector<double> d((size_t)nc), v((size_t)nc);

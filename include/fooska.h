@@ -147,7 +147,8 @@ int fk_set_clock(fk_ctx *ctx, double dt, double time, int64_t step);
  *  "grad"   [nElem][NEQ][3]   Green-Gauss gradients (needs fk_set_debug(1))
  *  "theta"  [nElem][NEQ]      limiter (needs fk_set_debug(1))
  *  "face"   [nElem][4][NEQ]   reconstructed face states per cell face slot (f)
- *  "rhs"    [nElem][NEQ]      residual of the last fk_compute_rhs (f)
+ *  "rhs"    [nElem][NEQ]      residual of the last fk_compute_rhs (f); mixed
+ *                             precision stores it as FP32 (widened here)
  *  "slots"  [nElem][4] int32  face id of each cell face slot
  *  "signal" [nElem]           max signal speed per cell                      */
 int fk_download(fk_ctx *ctx, const char *name, void *host, int64_t bytes);
