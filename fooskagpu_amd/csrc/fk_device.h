@@ -142,16 +142,59 @@ __device__ __forceinline__ void cons2prim(const Mat<double> &m, const double *U,
   }
 }
 
-// FP32 primitive record of a state held in registers: the values pass G
-// would form from U itself, (float) cons2prim(u), written by the kernels that
-// produce a new state so that the next pass G gathers 4 NEQ bytes per
-// neighbour instead of 8 NEQ (mixed precision).
+// FP64 reciprocal: hardware estimate plus one Newton step (relative error
+// near 1e-16, not correctly rounded)
+__device__ __forceinline__ double rcp_nr(double x) {
+  const double r = __builtin_amdgcn_rcp(x);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+
+// The primitive variables of a state rounded to FP32: the first-order states
+// of the mixed path.  cons2prim's quantities from reciprocals (3 instead of 7
+// FP64 divisions two-phase, 1 instead of 4 single-phase), still in FP64 (the
+// pressures cancel against gamma pInf): they differ from cons2prim by a few
+// FP64 ulps, far below the FP32 rounding, and keep the register footprint of
+// the kernels that form them small.  rho_m = U1 + U2 (= aL rhoL + aG rhoG) and
+// rho_k e_k = U_{6,7} / a_k.
+template <int NEQ>
+__device__ __forceinline__ void prims_f32(const Mat<double> &m, const double *__restrict__ U, float *s) {
+  if constexpr (NEQ == 5) {
+    const double ir = rcp_nr(U[0]);
+    const double u = U[1] * ir, v = U[2] * ir, w = U[3] * ir;
+    const double e = U[4] * ir - 0.5 * (u * u + v * v + w * w);
+    s[0] = (float)U[0];
+    s[1] = (float)u;
+    s[2] = (float)v;
+    s[3] = (float)w;
+    s[4] = (float)p1_of(m, U[0], e);
+  } else {
+    const double a = clamp_avf(m, U[0]);
+    const double ia = rcp_nr(a), ig = rcp_nr(1.0 - a);
+    const double rl = U[1] * ia, rg = U[2] * ig;
+    const double rm = U[1] + U[2], irm = rcp_nr(rm);
+    s[0] = (float)a;
+    s[1] = (float)rl;
+    s[2] = (float)rg;
+    s[3] = (float)(U[3] * irm);
+    s[4] = (float)(U[4] * irm);
+    s[5] = (float)(U[5] * irm);
+    s[6] = (float)((m.g[0] - 1) * (U[6] * ia) - m.g[0] * m.pi[0] - (m.g[0] - 1) * m.eta[0] * rl);
+    s[7] = (float)((m.g[1] - 1) * (U[7] * ig) - m.g[1] * m.pi[1] - (m.g[1] - 1) * m.eta[1] * rg);
+    s[8] = (float)U[9];
+    s[9] = (float)rm;
+  }
+}
+
+// FP32 primitive record of a state held in registers, written by the kernels
+// that produce a new state so that the next pass G gathers 4 NEQ bytes per
+// neighbour instead of 8 NEQ (mixed precision); the same values as pass G
+// forms from U itself.
 template <int NEQ>
 __device__ __forceinline__ void store_prims(const Mat<double> &m, const double *u, float *__restrict__ W) {
-  double t[NEQ];
-  cons2prim<NEQ>(m, u, t);
+  float t[NEQ];
+  prims_f32<NEQ>(m, u, t);
 #pragma unroll
-  for (int v = 0; v < NEQ; ++v) W[v] = (float)t[v];
+  for (int v = 0; v < NEQ; ++v) W[v] = t[v];
 }
 
 // ---------------- boundary ghost (DP bc.h, SP bc.h) ----------------

@@ -93,9 +93,9 @@ __device__ __forceinline__ int link_of(T w) {
 // 48M cells); mixed precision runs at the compiler's choice (3 waves per
 // SIMD; 2 and 4 measured no faster).
 //
-// FROMW (mixed precision): the first-order states come from the FP32
-// primitive records Wp = (float) cons2prim(U) written by the producer of U
-// (same values, half the gathered bytes, no FP64 conversions here: 6.45
+// Mixed precision: the first-order states are fk::prims_f32(U).  FROMW: they
+// come from the FP32 primitive records Wp written by the producer of U (the
+// same values, half the gathered bytes, no FP64 conversions here: 6.45
 // against 7.8 ms at 48M cells).
 template <int NEQ, typename T, bool VISC, bool HEAT, bool FROMW>
 __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
@@ -111,6 +111,13 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
   if constexpr (FROMW) {
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) W[v] = (T)Wp[c * NEQ + v];
+  } else if constexpr (sizeof(T) == 4) {
+    float Wf[NEQ];
+    double Uc[NEQ];
+    fk::load_rec<NEQ>(U + c * NEQ, Uc);
+    fk::prims_f32<NEQ>(md, Uc, Wf);
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) W[v] = Wf[v];
   } else {
     double Wd[NEQ];
     fk::cons2prim<NEQ>(md, U + c * NEQ, Wd);
@@ -142,6 +149,13 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
       if constexpr (FROMW) {
 #pragma unroll
         for (int v = 0; v < NEQ; ++v) Wn[v] = (T)Wp[(int64_t)nb * NEQ + v];
+      } else if constexpr (sizeof(T) == 4) {
+        float t[NEQ];
+        double Un[NEQ];
+        fk::load_rec<NEQ>(U + (int64_t)nb * NEQ, Un);
+        fk::prims_f32<NEQ>(md, Un, t);
+#pragma unroll
+        for (int v = 0; v < NEQ; ++v) Wn[v] = t[v];
       } else {
         double t[NEQ];
         fk::cons2prim<NEQ>(md, U + (int64_t)nb * NEQ, t);
