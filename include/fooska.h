@@ -109,14 +109,18 @@ int fk_init_rk(fk_ctx *ctx);
 int fk_compute_face_values(fk_ctx *ctx);
 /* computeRHS (computeRHS.cu:214): HLLC + non-conservative terms + pressure
  * correction (+ viscous/WALE) gathered per cell; kept on the device for
- * fk_update.  Also records the per-cell signal speed for fk_compute_time_step. */
+ * fk_update (FP64; FP32 in mixed precision, as the MP reference stores its
+ * RHS, promoted to FP64 in fk_update).  Also records the per-cell signal
+ * speed for fk_compute_time_step. */
 int fk_compute_rhs(fk_ctx *ctx);
 /* computeTimeStep (computeTimeStep.cu:172): dt = min(cfl * min_c V^0.3333 /
  * max signal, maxDt) from the face states of the last fk_compute_rhs. */
 int fk_compute_time_step(fk_ctx *ctx, double *dt);
 /* update (update.cu:105): RK stage irk with the residual of fk_compute_rhs. */
 int fk_update(fk_ctx *ctx, int irk, double dt);
-/* computeRHS + update fused: same result as fk_compute_rhs + fk_update. */
+/* computeRHS + update fused: same result as fk_compute_rhs + fk_update (in
+ * mixed precision up to the FP32 rounding of the stored residual, which the
+ * fused form keeps in FP64 registers). */
 int fk_compute_rhs_update(fk_ctx *ctx, int irk, double dt);
 /* relaxation (relaxation.cu:77): per-cell phase relaxation, mode FK_RELAX_*. */
 int fk_relaxation(fk_ctx *ctx, int mode, fk_relax_stats *stats);
