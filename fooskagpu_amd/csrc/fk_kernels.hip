@@ -520,8 +520,16 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
       Unew[c * NEQ + v] = u;
       un[v] = u;
     }
-    // mixed precision: FP32 primitive record of the new state for pass G
-    if constexpr (WOUT) fk::store_prims<NEQ>(md, un, Wout + c * NEQ);
+    // mixed precision: FP32 primitive record of the new state for pass G.  The
+    // EOS constants are re-derived from the (scalar) parameters here: shared
+    // with the cell pressure above, they would otherwise be held in vector
+    // registers across the face loop and spill.
+    if constexpr (WOUT) {
+      fk::Mat<double> mw = md;
+      asm volatile("" : "+s"(mw.g[0]), "+s"(mw.g[1]), "+s"(mw.pi[0]), "+s"(mw.pi[1]), "+s"(mw.eta[0]), "+s"(mw.eta[1]),
+                   "+s"(mw.avfMin));
+      fk::store_prims<NEQ>(mw, un, Wout + c * NEQ);
+    }
   }
 }
 
