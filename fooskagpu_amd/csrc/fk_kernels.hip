@@ -282,16 +282,33 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   T *sF = reinterpret_cast<T *>(lds_raw);
   T *sG = sF + kBlock * 4 * NEQ;
   if constexpr (STAGE) {
-    // each thread copies its own cell's records (measured faster here than a
-    // coalesced copy of the workgroup's contiguous block: 6.7 vs 7.5 ms)
-    if (c < nc) {
-      const T *src = face + c * 4 * NEQ;
-      T *dst = sF + threadIdx.x * 4 * NEQ;
+    // the workgroup's face records and gradients are contiguous in HBM and in
+    // LDS: copied by direct-to-LDS loads (global_load_lds: lane i of a wave
+    // fills bytes [s i, s i + s) after the wave's LDS base, s = 16 or 4), no
+    // registers involved.  Measured 7.0 against 7.3 ms (fused pass F, 48M
+    // cells) for a copy through registers, one cell per thread.
+    typedef __attribute__((address_space(3))) void *lds_ptr;
+    const int64_t ncell = (nc - c0) < kBlock ? (nc - c0) : kBlock;
+    const int wbase = __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
+    {
+      constexpr int kW = kBlock * 4 * NEQ * (int)sizeof(T) / 16;  // 16-byte words of a full block
+      const int nw = (int)(ncell * 4 * NEQ * (int64_t)sizeof(T) / 16);
+      const uint4 *src = reinterpret_cast<const uint4 *>(face + c0 * 4 * NEQ);
+      uint4 *dst = reinterpret_cast<uint4 *>(sF);
 #pragma unroll
-      for (int v = 0; v < 4 * NEQ; ++v) dst[v] = src[v];
-      if constexpr (VISC) {
+      for (int k = 0; k < kW / kBlock; ++k) {
+        const int i = k * kBlock + (int)threadIdx.x;
+        if (i < nw) __builtin_amdgcn_global_load_lds((const void *)(src + i), (lds_ptr)(dst + k * kBlock + wbase), 16, 0, 0);
+      }
+    }
+    if constexpr (VISC) {
+      const int nw = (int)(ncell * NGS);
+      const float *src = reinterpret_cast<const float *>(gvel + c0 * NGS);
+      float *dst = reinterpret_cast<float *>(sG);
 #pragma unroll
-        for (int v = 0; v < NGS; ++v) sG[threadIdx.x * NGS + v] = gvel[c * NGS + v];
+      for (int k = 0; k < NGS; ++k) {
+        const int i = k * kBlock + (int)threadIdx.x;
+        if (i < nw) __builtin_amdgcn_global_load_lds((const void *)(src + i), (lds_ptr)(dst + k * kBlock + wbase), 4, 0, 0);
       }
     }
     __syncthreads();
