@@ -1192,7 +1192,7 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
     if (other >= 0) {
       hlink[(size_t)s] = (fslot[(size_t)(2 * f + 1 - side)] << 1) | (side == 0 ? 1 : 0);
     } else {
-      if (boundCond[f] < 1) return fail(FK_EINVAL, "boundary face without a boundary condition (boundCond < 1)");
+      if (boundCond[f] < 1) return fail(FK_EINVAL, "boundCond must be >= 1 on boundary faces");
       hlink[(size_t)s] = -boundCond[f];
     }
     c->h_r[(size_t)(s * 4 + 3)] = (double)hlink[(size_t)s];

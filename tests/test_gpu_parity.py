@@ -248,3 +248,17 @@ def test_primitive_records_of_relaxation(gpu, name, mode):
     print(name, mode, st)
     assert np.isfinite(out[0]).all()
     assert np.array_equal(out[1], out[0])
+
+
+@pytest.mark.parametrize("bad", [0, -2])
+def test_boundary_face_without_condition_is_rejected(gpu, bad):
+    """A boundary face (no neighbour) needs a condition code >= 1: the slot
+    link stores -bc there, and 0 or a negative code would read as an
+    interior link."""
+    from fooskagpu_amd._lib import FkError
+    c = SMALL["sod"]()  # a fresh mesh: its host arrays can be edited
+    bnd = np.flatnonzero(c.mesh.fc2el[:, 1] < 0)
+    assert len(bnd) > 0
+    c.mesh.bc[bnd[0]] = bad
+    with pytest.raises(FkError, match="boundCond"):
+        GpuSolver(c.params, c.mesh, c.U)
