@@ -12,6 +12,7 @@ c_double_p = ctypes.POINTER(ctypes.c_double)
 c_int64_p = ctypes.POINTER(ctypes.c_int64)
 
 FK_OK = 0
+EINVAL = -1  # FK_EINVAL (include/fooska_mesh.h)
 MODEL_SINGLE = 0
 MODEL_TWO_PHASE = 1
 FP64 = 0
@@ -29,7 +30,9 @@ class FkParams(ctypes.Structure):
         "chemItrMax", "hllcEnergy", "relaxFlag", "relaxMode")] + [(n, ctypes.c_double) for n in (
         "dt", "cfl", "maxDt", "avfMin", "thermUrf", "chemUrf", "chemLimit", "satTempUrf",
         "polishTol")] + [(n, ctypes.c_double * 2) for n in (
-        "gamma", "pInf", "Cp", "eta", "satConst", "dynVisc", "Pr")]
+        "gamma", "pInf", "Cp", "eta", "satConst", "dynVisc", "Pr")] + [
+        ("thermFntRlxFlag", ctypes.c_int32), ("thermRlxRate", ctypes.c_double),
+        ("inletP", ctypes.c_double), ("inletT", ctypes.c_double)]
 
     @classmethod
     def from_dict(cls, d):

@@ -22,6 +22,7 @@ template <typename T>
 struct Mat {
   T g[2], pi[2], cp[2], eta[2];
   T avfMin, chemLimit, mu[2], Pr0;
+  T inP, inT;  // inlet ghost pressure and temperature
   int woods, hllcToro;
 };
 
@@ -38,6 +39,8 @@ __host__ __device__ inline Mat<T> make_mat(const fk_params &p) {
   m.avfMin = (T)p.avfMin;
   m.chemLimit = (T)p.chemLimit;
   m.Pr0 = (T)p.Pr[0];
+  m.inP = (T)p.inletP;
+  m.inT = (T)p.inletT;
   m.woods = p.woodsFlag;
   m.hllcToro = p.hllcEnergy;
   return m;
@@ -209,12 +212,13 @@ __device__ __forceinline__ void ghost(const Mat<T> &m, int bc, const T n[3], T *
 #pragma unroll
     for (int i = 0; i < 3; ++i) s[iu + i] = -s[iu + i];
   } else if (bc == 3) {
+    // inlet pressure and temperature: fk_params inletP / inletT, the
+    // reference's constants (bc.h:5-7) filled in when both are 0
+    const T P = m.inP, Tt = m.inT;
     if constexpr (NEQ == 5) {
-      T P = (T)(4.2 * 101300), Tt = (T)288;
       s[0] = rho_of(m, 0, P, Tt);
       s[4] = P;
     } else {
-      T P = (T)11.4e5, Tt = (T)115.3;
       s[0] = (T)1.0 - m.avfMin;
       s[1] = rho_of(m, 0, P, Tt);
       s[2] = rho_of(m, 1, P, Tt);

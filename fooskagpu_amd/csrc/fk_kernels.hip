@@ -1033,6 +1033,7 @@ int do_relax(fk_ctx *c, fk_relax_stats *st, int mode = FK_RELAX_PLAIN) {
   if (c->p.model != FK_MODEL_TWO_PHASE) return fail(FK_EINVAL, "relaxation needs the two-phase model");
   unsigned long long *cnt = (unsigned long long *)c->lcount;
   if (mode == FK_RELAX_MLP || mode == FK_RELAX_COLD) {
+    if (c->p.thermFntRlxFlag == 1) return fail(FK_EINVAL, "finite-rate relaxation needs FK_RELAX_PLAIN");
     int rc = fk_relax_mlp_impl(c, mode == FK_RELAX_MLP);
     if (rc) return rc;
   } else {
@@ -1077,7 +1078,25 @@ int check_params(const fk_params *p) {
   if (p->rkSteps != 1 && p->rkSteps != 3) return fail(FK_EINVAL, "rkSteps must be 1 or 3");
   if (p->adaptDt == 1 && p->adaptDtItr < 1) return fail(FK_EINVAL, "adaptDtItr must be >= 1");
   if (p->relaxMode < FK_RELAX_PLAIN || p->relaxMode > FK_RELAX_COLD) return fail(FK_EINVAL, "bad relaxMode");
+  if (p->thermFntRlxFlag != 0 && p->thermFntRlxFlag != 1) return fail(FK_EINVAL, "thermFntRlxFlag must be 0 or 1");
+  if (p->thermFntRlxFlag == 1 && p->relaxMode != FK_RELAX_PLAIN)
+    return fail(FK_EINVAL, "finite-rate relaxation needs relaxMode FK_RELAX_PLAIN");
+  const bool inlet_default = p->inletP == 0 && p->inletT == 0;
+  if (!inlet_default && !(p->inletT > 0 && std::isfinite(p->inletP) && std::isfinite(p->inletT)))
+    return fail(FK_EINVAL, "inletT must be > 0 and inletP finite (or both 0 for the default inlet)");
   return FK_OK;
+}
+
+// The parameters as the kernels use them: the inlet state filled in with the
+// reference's constants (bc.h:5-7 of either model) when both fields are 0.
+fk_params resolved(const fk_params &p) {
+  fk_params r = p;
+  if (r.inletP == 0 && r.inletT == 0) {
+    const bool sp = r.model == FK_MODEL_SINGLE;
+    r.inletP = sp ? 4.2 * 101300 : 11.4e5;
+    r.inletT = sp ? 288.0 : 115.3;
+  }
+  return r;
 }
 
 }  // namespace
@@ -1098,7 +1117,7 @@ int fk_create(int device, const fk_params *p, fk_ctx **out) {
   fk_ctx *c = new (std::nothrow) fk_ctx;
   if (!c) return fail(FK_ENOMEM, "out of host memory");
   c->device = device;
-  c->p = *p;
+  c->p = resolved(*p);
   c->neq = p->model == FK_MODEL_SINGLE ? 5 : 10;
   hipError_t e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking);
   if (e != hipSuccess) {
@@ -1133,7 +1152,7 @@ int fk_set_params(fk_ctx *c, const fk_params *p) {
   HIPCHK(hipSetDevice(c->device));
   const bool prec_changed = p->precision != c->p.precision;
   const bool heat_changed = (p->heatFlux != c->p.heatFlux) || (p->viscFlag != c->p.viscFlag);
-  c->p = *p;
+  c->p = resolved(*p);
   c->w_valid = false;
   if (c->have_mesh && (prec_changed || heat_changed)) {
     // the T-typed arrays were sized for double and for the largest NG; only refill geometry

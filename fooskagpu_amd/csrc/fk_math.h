@@ -1,4 +1,5 @@
-// fk_math.h -- portable FP64 logarithm shared by host and device code.
+// fk_math.h -- portable FP64 logarithm and exponential shared by host and
+// device code.
 //
 // The relaxation Newton iterations take finite-difference Jacobians with
 // steps of 1e-3 .. 1e-10 relative, which magnify a one-ulp difference in a
@@ -60,6 +61,39 @@ FK_HD inline double log(double x) {
 FK_HD inline double log10(double x) {
   const double inv_ln10 = 0x1.bcb7b1526e50ep-2;  // nearest double to 1 / ln 10
   return fkmath::log(x) * inv_ln10;
+}
+
+// exp(x) for the finite-rate relaxation factor, the same bits on the CPU and
+// on gfx950 for the same reason as log.  Method: k = nearest integer to
+// x / ln 2, r = (x - k ln2_hi) - k ln2_lo (k ln2_hi exact, |r| <= ln2 / 2 up to
+// rounding), exp(r) by its Taylor series through r^13 / 13! (truncation below
+// 4e-18 relative for |r| <= 0.347) in Horner form, and exp(x) = 2^k exp(r) by
+// an exact ldexp.  x < -708 (exp(x) < 2^-1021) returns 0, so no result is
+// subnormal; x > 709.78 (beyond the double range) returns +inf.
+FK_HD inline double exp(double x) {
+  if (x != x) return x;
+  if (x > 709.78) return HUGE_VAL;
+  if (x < -708.0) return 0.0;
+  const double inv_ln2 = 0x1.71547652b82fep0;  // nearest double to 1 / ln 2
+  const double ln2_hi = 0x1.62e42ffp-1;
+  const double ln2_lo = -0x1.718432a1b0e26p-35;
+  const double kd = rint(x * inv_ln2);
+  const double r = (x - kd * ln2_hi) - kd * ln2_lo;
+  double q = 1.0 / 6227020800.0;  // 1 / 13!
+  q = q * r + 1.0 / 479001600.0;
+  q = q * r + 1.0 / 39916800.0;
+  q = q * r + 1.0 / 3628800.0;
+  q = q * r + 1.0 / 362880.0;
+  q = q * r + 1.0 / 40320.0;
+  q = q * r + 1.0 / 5040.0;
+  q = q * r + 1.0 / 720.0;
+  q = q * r + 1.0 / 120.0;
+  q = q * r + 1.0 / 24.0;
+  q = q * r + 1.0 / 6.0;
+  q = q * r + 0.5;
+  q = q * r + 1.0;
+  q = q * r + 1.0;
+  return ldexp(q, (int)kd);
 }
 
 }  // namespace fkmath

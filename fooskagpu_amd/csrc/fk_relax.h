@@ -46,9 +46,45 @@ FK_HD inline double tsat_dev(const fk_params &P, double Pr, double temp, double 
   return T;
 }
 
+// Finite-rate targets of the thermal relaxation (fntPressRlx,
+// thermRlx.cu:42-67).  With the model on, the Newton solve drives
+// p_L - p_G to dP and T_L - T_G to dT instead of to zero, where
+//   dP = (p_L - p_G) exp(-|kappa| dt),  kappa = alpha_G m sum_k rho_k c_k / alpha_k,
+//   c_k = (p_k / rho_k^2 - de_k/drho_k) / (de_k/dp_k)   (= gamma_k (p_k + pinf_k) / rho_k,
+//         the squared sound speed),
+//   dT = theta (T_L - T_G),  theta = dP / (p_L - p_G) if p_L - p_G > 1e-5, else 0,
+// all from the cell's state after the energy correction, once per cell.  The
+// precisions of the reference are kept: p_k, T_k, the derivatives, c_k, the
+// rate m and dP, dT, theta are FP32 values; kappa and the exponential are
+// FP64.  dt is fk_params.dt, the configured step (the reference reads
+// d_input->dt, which the adaptive time step never updates).  theta's FP32
+// quotient is taken as the FP64 quotient rounded to FP32, which is the
+// correctly rounded FP32 quotient (53 >= 2 * 24 + 2 bits).
+FK_HD inline void fnt_press_rlx(const fk_params &P, const double avf[2], const double rho[2], const double ener[2],
+                                float *dP, float *dT) {
+  const float m = (float)P.thermRlxRate;
+  float press[2], temp[2];
+  double kappa = 0;
+  for (int k = 0; k < 2; ++k) {
+    press[k] = (float)eos_p(P, k, rho[k], ener[k]);
+    const double g = P.gamma[k];
+    const float de_drho = (float)(-((g * P.pInf[k] + (double)press[k]) / ((g - 1) * (rho[k] * rho[k]))));
+    const float de_dp = (float)(1 / ((g - 1) * rho[k]));
+    const float c = (float)(((double)press[k] / (rho[k] * rho[k]) - (double)de_drho) / (double)de_dp);
+    kappa += rho[k] * (double)c / avf[k];
+  }
+  kappa = kappa * avf[1] * (double)m;
+  const float dp = press[0] - press[1];
+  *dP = (float)((double)dp * fkmath::exp(-fabs(kappa) * P.dt));
+  for (int k = 0; k < 2; ++k) temp[k] = (float)eos_T(P, k, rho[k], ener[k]);
+  const float theta = ((double)dp > 1e-5) ? (float)((double)*dP / (double)dp) : 0.0f;
+  *dT = theta * (temp[0] - temp[1]);
+}
+
 struct ThermSys {
   int pm;
   double ar[2], enr[2];
+  float dP, dT;  // finite-rate targets, 0 for equilibrium
   FK_HD void res(const fk_params &P, double x[4], double *r0, double *r1) const {
     const int k = 1 - pm;
     double s1 = ar[k] / x[k];
@@ -57,8 +93,8 @@ struct ThermSys {
     x[2 + pm] = enr[pm] - (1 / ar[pm]) * s2;
     double p0 = eos_p(P, 0, x[0], x[2]), p1 = eos_p(P, 1, x[1], x[3]);
     double t0 = eos_T(P, 0, x[0], x[2]), t1 = eos_T(P, 1, x[1], x[3]);
-    *r0 = p0 - p1 - (double)0.0f;
-    *r1 = t0 - t1 - (double)0.0f;
+    *r0 = p0 - p1 - (double)dP;
+    *r1 = t0 - t1 - (double)dT;
   }
 };
 
@@ -83,6 +119,12 @@ FK_HD inline int therm_dev(const fk_params &P, double *U) {
   const double de = U[8] / rhoMix - Y0 * t.enr[0] - Y1 * t.enr[1] - 0.5 * (v0 * v0 + v1 * v1 + v2 * v2);
   t.enr[0] += de;
   t.enr[1] += de;
+  t.dP = 0.0f;
+  t.dT = 0.0f;
+  if (P.thermFntRlxFlag == 1) {
+    const double avf[2] = {avf0, avf1}, rho[2] = {rho0, rho1};
+    fnt_press_rlx(P, avf, rho, t.enr, &t.dP, &t.dT);
+  }
   int abortFlag = 0;
   float minRes = 1e9f;
   int itr = 0;

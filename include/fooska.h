@@ -65,6 +65,18 @@ typedef struct fk_params {
   double thermUrf, chemUrf, chemLimit, satTempUrf;
   double polishTol;   /* relative tolerance of the FP64 Newton polish */
   double gamma[2], pInf[2], Cp[2], eta[2], satConst[2], dynVisc[2], Pr[2];
+  /* Finite-rate thermal relaxation (INPUT::thermFntRlxFlag / thermRlxRate,
+   * strdata.h:72-73; fntPressRlx, thermRlx.cu:42-67): 1 makes the thermal
+   * Newton solve target p_L - p_G = dP, T_L - T_G = dT with
+   * dP = (p_L - p_G) exp(-|kappa| dt) (dt = the `dt` field above) instead of
+   * equilibrium.  The reference's defaults are 0 and 0.5 (buildInput.cu:35-36).
+   * Only FK_RELAX_PLAIN supports it. */
+  int32_t thermFntRlxFlag;
+  double thermRlxRate;
+  /* Inlet (boundCond 3) ghost pressure and temperature (bc.h:5-7).  Both 0:
+   * the reference's constants, 11.4e5 Pa / 115.3 K (two-phase) and
+   * 4.2 * 101300 Pa / 288 K (single-phase). */
+  double inletP, inletT;
 } fk_params;
 
 typedef struct fk_ctx fk_ctx;

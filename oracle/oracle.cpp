@@ -112,12 +112,14 @@ void bc_ghost(const fk_params &p, int bc, const double *n, double *s) {
   } else if (bc == 5) {  // no-slip wall
     for (int i = 0; i < 3; ++i) s[iu + i] = -s[iu + i];
   } else if (bc == 3) {  // inlet (Dirichlet p, T; velocity kept)
+    // fk_params inletP / inletT; both 0 selects the constants of bc.h:5-7
+    const bool ref = p.inletP == 0 && p.inletT == 0;
     if (p.model == FK_MODEL_SINGLE) {
-      double P = 4.2 * 101300, T = 288;
+      double P = ref ? 4.2 * 101300 : p.inletP, T = ref ? 288 : p.inletT;
       s[0] = eqos1(p, 2, P, T);
       s[4] = P;
     } else {
-      double P = 11.4e5, T = 115.3;
+      double P = ref ? 11.4e5 : p.inletP, T = ref ? 115.3 : p.inletT;
       s[0] = 1.0 - p.avfMin;
       s[1] = eqos2(p, 0, 2, P, T);
       s[2] = eqos2(p, 1, 2, P, T);
@@ -437,6 +439,34 @@ struct Therm {
   }
 };
 
+// fntPressRlx (thermRlx.cu:42-67): finite-rate targets dP, dT of the thermal
+// residuals.  The reference's types are kept: `float` for the pressures,
+// temperatures, derivatives, c, m, theta, dP and dT, `double` for kappa and the
+// exponential (exp as fkmath::exp, bitwise the same on host and device).
+void fnt_press_rlx(const fk_params &p, const double avf[2], const double rho[2], const double ener[2], float *dP,
+                   float *dT) {
+  float de_drho[2], de_dP[2], p_I[2], c[2];
+  float m = (float)p.thermRlxRate;
+  float press[2];
+  press[0] = (float)eqos2(p, 0, 0, rho[0], ener[0]);
+  press[1] = (float)eqos2(p, 1, 0, rho[1], ener[1]);
+  double kappa = 0;
+  for (int iv = 0; iv < 2; iv++) {
+    de_drho[iv] = (float)(-((p.gamma[iv] * p.pInf[iv] + press[iv]) / ((p.gamma[iv] - 1) * (rho[iv] * rho[iv]))));
+    de_dP[iv] = (float)(1 / ((p.gamma[iv] - 1) * rho[iv]));
+    p_I[iv] = press[iv];
+    c[iv] = (float)((p_I[iv] / (rho[iv] * rho[iv]) - de_drho[iv]) / de_dP[iv]);
+    kappa += rho[iv] * c[iv] / avf[iv];
+  }
+  kappa = kappa * avf[1] * m;
+  *dP = (float)((press[0] - press[1]) * fkmath::exp(-std::fabs(kappa) * p.dt));
+  float temp[2];
+  temp[0] = (float)eqos2(p, 0, 1, rho[0], ener[0]);
+  temp[1] = (float)eqos2(p, 1, 1, rho[1], ener[1]);
+  float theta = (press[0] - press[1] > 1e-5) ? (*dP) / (press[0] - press[1]) : 0.0f;
+  *dT = theta * (temp[0] - temp[1]);
+}
+
 // thermalRlx for one cell (thermRlx.cu:189-235 with helpers :4-187)
 int therm_cell(const fk_params &p, double *U) {
   const int itrMax = p.thermItrMax;
@@ -473,6 +503,7 @@ int therm_cell(const fk_params &p, double *U) {
   double de = rE / rhoMix - Y[0] * t.enr[0] - Y[1] * t.enr[1] - 0.5 * (vel[0] * vel[0] + vel[1] * vel[1] + vel[2] * vel[2]);
   t.enr[0] += de;
   t.enr[1] += de;
+  if (p.thermFntRlxFlag == 1) fnt_press_rlx(p, avf, rho, t.enr, &t.dP, &t.dT);  // thermRlx.cu:213-214
   int abortFlag = 0;
   float minRes = 1e9f;
   bool conv = false;
@@ -1266,6 +1297,11 @@ double orc_wale(const double *g, double vol) {
 
 double orc_tsat(const fk_params *p, double P, double T0, double urf, int *iters) { return tsat(*p, P, T0, urf, iters); }
 double orc_log10(double x) { return fkmath::log10(x); }
+double orc_exp(double x) { return fkmath::exp(x); }
+void orc_fnt_press_rlx(const fk_params *p, const double *avf, const double *rho, const double *ener, float *dP,
+                       float *dT) {
+  fnt_press_rlx(*p, avf, rho, ener, dP, dT);
+}
 double orc_gibbs(const fk_params *p, int k, double P, double T) { return gibbs(*p, k, P, T); }
 
 int orc_relax_cell(const fk_params *p, double *U, int *iters, int *tsat_iters) {

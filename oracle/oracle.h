@@ -76,6 +76,11 @@ double orc_wale(const double *g, double vol);
 double orc_tsat(const fk_params *p, double P, double T0, double urf, int *iters);
 /* Shared portable log10 of the relaxation and the Gibbs energy g_k(P, T). */
 double orc_log10(double x);
+double orc_exp(double x);
+/* Finite-rate targets (fntPressRlx, thermRlx.cu:42-67) of phase volume
+ * fractions avf[2], densities rho[2] and internal energies ener[2]. */
+void orc_fnt_press_rlx(const fk_params *p, const double *avf, const double *rho, const double *ener, float *dP,
+                       float *dT);
 double orc_gibbs(const fk_params *p, int k, double P, double T);
 /* Relax one cell's conserved state in place; returns 0 thermal, 1 chemical;
  * iters = Newton iterations. */

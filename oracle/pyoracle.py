@@ -56,6 +56,9 @@ def lib(omp=False):
                                        ctypes.POINTER(ctypes.c_int)]),
         "orc_relax_cell": (ctypes.c_int, [P, _dp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]),
         "orc_log10": (ctypes.c_double, [ctypes.c_double]),
+        "orc_exp": (ctypes.c_double, [ctypes.c_double]),
+        "orc_fnt_press_rlx": (None, [P, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_float),
+                                     ctypes.POINTER(ctypes.c_float)]),
         "orc_gibbs": (ctypes.c_double, [P, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
     }
     for n, (r, a) in sig.items():
@@ -221,6 +224,19 @@ def tsat(params, P, T0, urf=0.8):
 def log10(x):
     """Portable log10 shared by the oracle and the device relaxation."""
     return lib().orc_log10(float(x))
+
+
+def exp(x):
+    """Portable exp shared by the oracle and the device relaxation."""
+    return lib().orc_exp(float(x))
+
+
+def fnt_press_rlx(params, avf, rho, ener):
+    """Finite-rate relaxation targets (dP, dT) as FP32 values."""
+    dP, dT = ctypes.c_float(), ctypes.c_float()
+    a, r, e = (np.ascontiguousarray(v, dtype=np.float64) for v in (avf, rho, ener))
+    lib().orc_fnt_press_rlx(ctypes.byref(_P(params)), _d(a), _d(r), _d(e), ctypes.byref(dP), ctypes.byref(dT))
+    return np.float32(dP.value), np.float32(dT.value)
 
 
 def gibbs(params, k, P, T):

@@ -3,6 +3,7 @@
 import ctypes
 import os
 import re
+import subprocess
 
 import pytest
 
@@ -32,9 +33,18 @@ def test_python_bindings_cover_the_headers():
         assert set(declared(h)) <= bound, set(declared(h)) - bound
 
 
-def test_params_struct_layout_matches_header():
-    # 18 int32 + 9 double + 7 double[2]
-    assert ctypes.sizeof(_lib.FkParams) == 18 * 4 + 9 * 8 + 14 * 8
+def test_params_struct_layout_matches_header(tmp_path):
+    """Every FkParams field sits at the offset the C compiler gives it."""
+    names = [n for n, _ in _lib.FkParams._fields_]
+    src = tmp_path / "layout.c"
+    src.write_text('#include <stddef.h>\n#include <stdio.h>\n#include "fooska.h"\nint main(void) {\n'
+                   + "".join(f'  printf("%zu\\n", offsetof(fk_params, {n}));\n' for n in names)
+                   + '  printf("%zu\\n", sizeof(fk_params));\n  return 0;\n}\n')
+    exe = tmp_path / "layout"
+    subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    want = [getattr(_lib.FkParams, n).offset for n in names] + [ctypes.sizeof(_lib.FkParams)]
+    assert got == want
 
 
 def test_create_without_device_fails_cleanly():
@@ -46,3 +56,21 @@ def test_create_without_device_fails_cleanly():
     rc = L.fk_create(9999, ctypes.byref(p), ctypes.byref(h))
     assert rc < 0 and not h.value
     assert L.fk_last_error()
+
+
+@pytest.mark.parametrize("over,msg", [
+    (dict(thermFntRlxFlag=1, relaxMode=_lib.RELAX_MLP), "finite-rate"),
+    (dict(thermFntRlxFlag=2), "thermFntRlxFlag"),
+    (dict(inletP=2e6), "inletT"),
+    (dict(inletP=2e6, inletT=-5.0), "inletT"),
+])
+def test_create_rejects_bad_relaxation_and_inlet_parameters(over, msg):
+    """Parameter checks run before any device call: finite-rate relaxation
+    only with the plain Newton relaxation, an inlet needs T > 0."""
+    from fooskagpu_amd import cases
+    L = _lib.hip_lib()
+    p = _lib.FkParams.from_dict(dict(cases._two_phase_params(), **over))
+    h = ctypes.c_void_p()
+    rc = L.fk_create(0, ctypes.byref(p), ctypes.byref(h))
+    assert rc == _lib.EINVAL and not h.value
+    assert msg in L.fk_last_error().decode()

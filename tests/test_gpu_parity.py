@@ -262,3 +262,91 @@ def test_boundary_face_without_condition_is_rejected(gpu, bad):
     c.mesh.bc[bnd[0]] = bad
     with pytest.raises(FkError, match="boundCond"):
         GpuSolver(c.params, c.mesh, c.U)
+
+
+# ---------------- finite-rate relaxation and the inlet state ----------------
+FNT = dict(thermFntRlxFlag=1, thermRlxRate=0.5, dt=1e-9)  # exp(-kappa dt) spans (0, 1) over the cells
+
+
+@pytest.mark.parametrize("name", ["bubble", "shock_bubble"])
+def test_finite_rate_relaxation_matches_oracle(gpu, name):
+    """fntPressRlx targets in the thermal Newton solve (thermRlx.cu:42-67,
+    213-214): FP64 path against the oracle after 10 steps, and the model
+    changes the state well above that bar (so the comparison tests it)."""
+    c = SMALL[name]()
+    p = dict(c.params, **FNT)
+    o = Oracle(p, c.mesh, c.U)
+    o.run_steps(10)
+    Uo = o.state()
+    with GpuSolver(p, c.mesh, c.U) as g:
+        g.run_steps(10)
+        Ug = g.download_state()
+    with GpuSolver(c.params, c.mesh, c.U) as g:
+        g.run_steps(10)
+        Ueq = g.download_state()
+    err = rel_l1(Ug, Uo)
+    eff = rel_l1(Ug, Ueq)
+    print(f"[{name}] finite-rate FP64 rel L1 {err.max():.2e}; change from equilibrium {eff.max():.2e}", flush=True)
+    assert np.isfinite(Ug).all() and err.max() <= FP64_TOL
+    assert eff.max() > 1e-6
+
+
+def test_finite_rate_relaxation_mixed_precision(gpu):
+    c = SMALL["shock_bubble"]()
+    p = dict(c.params, **FNT)
+    o = Oracle(p, c.mesh, c.U)
+    o.run_steps(10)
+    Uo = o.state()
+    with GpuSolver(dict(p, precision=_lib.MIXED), c.mesh, c.U) as g:
+        g.run_steps(10)
+        Ug = g.download_state()
+    rho_err = rel_l1(Ug[:, [1, 2]].sum(1, keepdims=True), Uo[:, [1, 2]].sum(1, keepdims=True))[0]
+    assert rho_err <= MIXED_RHO_TOL
+
+
+def inlet_case(model, **inlet):
+    """Box with an inlet (boundCond 3) at x = 0, outlet at x = 1, slip walls."""
+    from fooskagpu_amd import Mesh
+    from fooskagpu_amd.mesh import BC_INLET, BC_OUTLET, BC_SLIP
+    bc = (BC_INLET, BC_OUTLET) + (BC_SLIP,) * 4
+    if model == _lib.MODEL_SINGLE:
+        c = cases.sod(nx=16, ny=4, nz=4)
+        m = cases.Mesh.box(16, 4, 4, (0, 0, 0), (1.0, 0.25, 0.25), 0.0, 0, bc=bc)
+        U = m.init_sod(1.4, 0.0, 0.5, (1.0, 0, 0, 0, 1.0), (0.125, 0, 0, 0, 0.1))
+        return cases.Case("sod_inlet", dict(c.params, **inlet), m, U, 10)
+    p = cases._two_phase_params(**inlet)
+    m = Mesh.box(10, 6, 6, (0, 0, 0), (1, 0.6, 0.6), 0.10, 3, bc=bc)
+    am = p["avfMin"]
+    regions = [dict(kind=0, alphaL=1 - am, p=1e7, T=cases.T_WATER, u=(5.0, 0, 0)),
+               dict(kind=1, c=(0.5, 0.3, 0.3), r=0.15, alphaL=am, p=2340.0, T=cases.T_WATER)]
+    return cases.Case("bubble_inlet", p, m, m.init_regions(p, regions), 10)
+
+
+@pytest.mark.parametrize("model,inlet", [(_lib.MODEL_TWO_PHASE, {}),
+                                         (_lib.MODEL_TWO_PHASE, dict(inletP=1.2e7, inletT=300.0)),
+                                         (_lib.MODEL_SINGLE, dict(inletP=1.2, inletT=0.004))])
+def test_inlet_boundary_matches_oracle(gpu, model, inlet):
+    """Inlet ghost (bc.h:2-25) with the reference's constants or a configured
+    pressure and temperature: FP64 against the oracle after 10 steps; mixed
+    precision within its density bar."""
+    c = inlet_case(model, **inlet)
+    o = Oracle(c.params, c.mesh, c.U)
+    o.run_steps(10)
+    Uo = o.state()
+    with GpuSolver(c.params, c.mesh, c.U) as g:
+        g.run_steps(10)
+        Ug = g.download_state()
+    with GpuSolver(dict(c.params, precision=_lib.MIXED), c.mesh, c.U) as g:
+        g.run_steps(10)
+        Um = g.download_state()
+    err = rel_l1(Ug, Uo)
+    cols = neq_density(c)
+    rho_err = rel_l1(Um[:, cols].sum(1, keepdims=True), Uo[:, cols].sum(1, keepdims=True))[0]
+    print(f"[{c.name} {inlet}] FP64 rel L1 {err.max():.2e}; mixed density {rho_err:.2e}", flush=True)
+    assert np.isfinite(Ug).all() and err.max() <= FP64_TOL
+    assert rho_err <= (MIXED_RHO_TOL if c.neq == 10 else MIXED_RHO_TOL_SINGLE)
+    # the inlet state reaches the solution: another inlet pressure changes it
+    o2 = Oracle(dict(c.params, inletP=2 * c.params.get("inletP", 11.4e5), inletT=c.params.get("inletT", 115.3)),
+                c.mesh, c.U)
+    o2.run_steps(10)
+    assert rel_l1(o2.state(), Uo).max() > 1e-3

@@ -414,3 +414,120 @@ def test_gibbs_energy_closed_form():
             s = (p["Cp"][k] / p["gamma"][k]) * math.log10(T ** p["gamma"][k] / (P + p["pInf"][k]) ** (p["gamma"][k] - 1))
             g = p["Cp"][k] * T + p["eta"][k] - T * (s + et)
             assert po.gibbs(p, k, P, T) == pytest.approx(g, rel=1e-12, abs=1e-6)
+
+
+def test_portable_exp_within_one_ulp_of_libm():
+    """fkmath::exp (the finite-rate relaxation factor, shared by the oracle and
+    the device) against the C library over the range where exp is a normal
+    double, near 0, and at the special cases."""
+    rng = np.random.default_rng(12)
+    xs = np.concatenate([rng.uniform(-708, 709.7, 20000), rng.uniform(-1, 1, 5000), rng.uniform(-1e-8, 1e-8, 1000),
+                         [0.5, 1.0, -1.0, 2.0, 100.0, -100.0, 709.0, -707.9]])
+    worst = max(abs(po.exp(x) - math.exp(x)) / math.ulp(math.exp(x)) for x in xs)
+    assert worst <= 1.0, worst
+    assert po.exp(0.0) == 1.0 and po.exp(-0.0) == 1.0
+    assert po.exp(-800.0) == 0.0 and po.exp(-math.inf) == 0.0
+    assert po.exp(710.0) == math.inf and po.exp(math.inf) == math.inf
+    assert math.isnan(po.exp(math.nan))
+
+
+def fnt_closed_form(params, avf, rho, ener):
+    """fntPressRlx (thermRlx.cu:42-67) in closed form: with de/drho =
+    -(gamma pinf + p) / ((gamma - 1) rho^2) and de/dp = 1 / ((gamma - 1) rho),
+    c_k = (p / rho^2 - de/drho) / (de/dp) = gamma (p + pinf) / rho."""
+    g, pi = params["gamma"], params["pInf"]
+    p = [po.eqos(params, k, 0, rho[k], ener[k]) for k in (0, 1)]
+    T = [po.eqos(params, k, 1, rho[k], ener[k]) for k in (0, 1)]
+    kappa = avf[1] * params["thermRlxRate"] * sum(rho[k] * g[k] * (p[k] + pi[k]) / rho[k] / avf[k] for k in (0, 1))
+    dP = (p[0] - p[1]) * math.exp(-abs(kappa) * params["dt"])
+    theta = dP / (p[0] - p[1]) if p[0] - p[1] > 1e-5 else 0.0
+    return dP, theta * (T[0] - T[1]), kappa
+
+
+@pytest.mark.parametrize("aL,pL,pG,TL,TG", [(0.7, 2e6, 1e6, 350.0, 340.0), (0.3, 3e5, 1e5, 300.0, 320.0),
+                                            (0.999, 1e7, 2340.0, 293.15, 293.15), (0.5, 5e5, 4e5, 400.0, 390.0)])
+def test_finite_rate_targets_closed_form(aL, pL, pG, TL, TG):
+    avf = [aL, 1 - aL]
+    rho = [po.eqos(TP, 0, 2, pL, TL), po.eqos(TP, 1, 2, pG, TG)]
+    ener = [po.eqos(TP, 0, 3, pL, rho[0]), po.eqos(TP, 1, 3, pG, rho[1])]
+    base = dict(TP, thermFntRlxFlag=1, thermRlxRate=0.5, dt=1.0)
+    _, _, kappa = fnt_closed_form(base, avf, rho, ener)
+    for kdt in (0.05, 1.0, 3.0):  # relaxation factor exp(-kappa dt) from 0.95 to 0.05
+        prm = dict(base, dt=kdt / kappa)
+        dP, dT = po.fnt_press_rlx(prm, avf, rho, ener)
+        dP_ref, dT_ref, _ = fnt_closed_form(prm, avf, rho, ener)
+        assert dP == pytest.approx(dP_ref, rel=1e-5), kdt
+        assert dT == pytest.approx(dT_ref, rel=1e-5, abs=1e-5), kdt
+    # dt = 0: no relaxation, the targets are the current differences (rounded to FP32)
+    dP, dT = po.fnt_press_rlx(dict(base, dt=0.0), avf, rho, ener)
+    assert dP == np.float32(np.float32(pL) - np.float32(pG))
+    assert dT == pytest.approx(TL - TG, rel=1e-5, abs=1e-4)
+
+
+def test_finite_rate_temperature_target_needs_liquid_overpressure():
+    """theta is dP / (p_L - p_G) only where p_L - p_G > 1e-5; otherwise the
+    temperature target is 0 (thermRlx.cu:65-66)."""
+    avf = [0.6, 0.4]
+    rho = [po.eqos(TP, 0, 2, 1e5, 320.0), po.eqos(TP, 1, 2, 3e5, 300.0)]
+    ener = [po.eqos(TP, 0, 3, 1e5, rho[0]), po.eqos(TP, 1, 3, 3e5, rho[1])]
+    prm = dict(TP, thermFntRlxFlag=1, thermRlxRate=0.5, dt=1e-9)
+    dP, dT = po.fnt_press_rlx(prm, avf, rho, ener)
+    dP_ref, _, _ = fnt_closed_form(prm, avf, rho, ener)
+    assert dP < 0 and dP == pytest.approx(dP_ref, rel=1e-5)
+    assert dT == 0.0
+
+
+def test_finite_rate_relaxation_reaches_its_targets():
+    """With the model on, the thermal Newton solve ends at p_L - p_G = dP and
+    T_L - T_G = dT (to the solver's tolerance) and conserves what the
+    equilibrium solve conserves; a rate so large that exp(-kappa dt) = 0 gives
+    the equilibrium solve bit for bit."""
+    for aL, pL, pG in [(0.7, 2e6, 1e6), (0.3, 1e5, 3e5), (0.5, 5e5, 4e5)]:
+        U = mixture_state(aL, pL, pG, 350.0)
+        off = dict(TP, chemFlag=0)
+        U_eq, _, _, _ = po.relax_cell(off, U)
+        U_huge, kind, _, _ = po.relax_cell(dict(off, thermFntRlxFlag=1, thermRlxRate=1e30, dt=1e-6), U)
+        assert kind == 0 and np.array_equal(U_huge, U_eq)
+        # relaxation factor exp(-kappa dt) = 0.5
+        a = np.clip(U[0], TP["avfMin"] / 100, 1 - TP["avfMin"] / 100)
+        avf, rho = [a, 1 - a], [U[1] / a, U[2] / (1 - a)]
+        rhoMix = U[1] + U[2]
+        de = U[8] / rhoMix - (U[6] + U[7]) / rhoMix - 0.5 * (U[3:6] @ U[3:6]) / rhoMix ** 2
+        ener = [U[6] / U[1] + de, U[7] / U[2] + de]
+        _, _, kappa = fnt_closed_form(dict(off, thermRlxRate=0.5, dt=1.0), avf, rho, ener)
+        prm = dict(off, thermFntRlxFlag=1, thermRlxRate=0.5, dt=math.log(2) / kappa)
+        dP, dT = po.fnt_press_rlx(prm, avf, rho, ener)
+        U2, kind, _, _ = po.relax_cell(prm, U)
+        assert kind == 0
+        W = po.cons2prim(TP, U2)
+        TLq = po.eqos(TP, 0, 1, W[1], U2[6] / (W[0] * W[1]))
+        TGs = po.eqos(TP, 1, 1, W[2], U2[7] / ((1 - W[0]) * W[2]))
+        assert abs((W[6] - W[7]) - dP) < 0.1 and abs((TLq - TGs) - dT) < 1e-2
+        assert abs(dP) > 1e3 and abs(dP - 0.5 * (pL - pG)) < 0.05 * abs(pL - pG)
+        assert np.array_equal(U2[[1, 2, 3, 4, 5, 8, 9]], U[[1, 2, 3, 4, 5, 8, 9]])
+        ke = 0.5 * (U[3:6] @ U[3:6]) / rhoMix
+        assert U2[6] + U2[7] == pytest.approx(U[8] - ke, rel=1e-12)
+
+
+@pytest.mark.parametrize("model", [0, 1])
+def test_inlet_ghost_takes_configured_state(model):
+    """Inlet (boundCond 3): fk_params inletP / inletT set the ghost's pressure
+    and temperature, rho_k = (p + pinf_k) / (Cv_k T (gamma_k - 1)); both 0
+    keep the constants of bc.h:5-7."""
+    params = SP if model == 0 else TP
+    n = np.array([1.0, 0.0, 0.0])
+    W = po.cons2prim(params, cases.sod().U[0]) if model == 0 else po.cons2prim(params, cases.bubble(n=2).U[0])
+    P, T = 3.3e6, 310.0
+    g = po.bc_ghost(dict(params, inletP=P, inletT=T), BC_INLET, n, W)
+    gam, pi, cp = params["gamma"], params["pInf"], params["Cp"]
+    rho = [(P + pi[k]) / (cp[k] / gam[k] * T * (gam[k] - 1)) for k in (0, 1)]
+    if model == 0:
+        assert g[4] == P and g[0] == pytest.approx(rho[0], rel=1e-14)
+        d = po.bc_ghost(params, BC_INLET, n, W)
+        assert d[4] == 4.2 * 101300 and d[0] == pytest.approx(4.2 * 101300 / (1006 / 1.4 * 288 * 0.4), rel=1e-14)
+    else:
+        assert g[6] == g[7] == P and g[0] == 1 - params["avfMin"] and g[8] == g[9] == 0
+        assert g[1] == pytest.approx(rho[0], rel=1e-14) and g[2] == pytest.approx(rho[1], rel=1e-14)
+        d = po.bc_ghost(params, BC_INLET, n, W)
+        assert d[6] == 11.4e5 and d[1] == pytest.approx((11.4e5 + pi[0]) / (cp[0] / gam[0] * 115.3 * (gam[0] - 1)),
+                                                         rel=1e-14)
