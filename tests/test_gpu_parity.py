@@ -350,3 +350,24 @@ def test_inlet_boundary_matches_oracle(gpu, model, inlet):
                 c.mesh, c.U)
     o2.run_steps(10)
     assert rel_l1(o2.state(), Uo).max() > 1e-3
+
+
+def test_finite_rate_with_huge_rate_is_the_equilibrium_solve(gpu):
+    """exp(-kappa dt) = 0 makes both finite-rate targets 0: the device run is
+    bitwise the equilibrium run."""
+    c = SMALL["bubble"]()
+    out = []
+    for extra in ({}, dict(thermFntRlxFlag=1, thermRlxRate=1e30, dt=1e-6)):
+        with GpuSolver(dict(c.params, **extra), c.mesh, c.U) as g:
+            g.run_steps(5)
+            out.append(g.download_state())
+    assert np.array_equal(out[0], out[1])
+
+
+def test_finite_rate_refuses_the_surrogate_relaxation(gpu):
+    from fooskagpu_amd._lib import FkError
+    c = SMALL["bubble"]()
+    with GpuSolver(dict(c.params, **FNT), c.mesh, c.U) as g:
+        with pytest.raises(FkError, match="finite-rate"):
+            g.relaxation(_lib.RELAX_MLP)
+        g.relaxation(_lib.RELAX_PLAIN)
