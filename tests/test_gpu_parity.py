@@ -1,7 +1,7 @@
 """GPU parity: the HIP path (through the C ABI) against the CPU FP64 oracle
 on the same seeded inputs.
 
-Bars (north star): FP64 path <= 1e-12 relative L1 per conserved variable
+Bars: FP64 path <= 1e-12 relative L1 per conserved variable
 after N steps; mixed path <= 1e-5 relative L1 in density.  Phase-level
 intermediates of the FP64 path are expected to agree to rounding (the
 arithmetic order is the same; only libm transcendentals in the relaxation
@@ -15,7 +15,7 @@ from oracle.pyoracle import Oracle
 pytestmark = pytest.mark.gpu
 
 FP64_TOL = 1e-12
-MIXED_RHO_TOL = 1e-5  # two-phase density
+MIXED_RHO_TOL = 1e-5  # two-phase density, the bar of the mixed path
 # Single-phase mixed: at Mach 0.1 (TGV) the FP32 face pressure p0 ~ 71 carries
 # an absolute rounding of ~4e-6 against dynamic pressure differences of ~0.06,
 # so the FP32-flux path is bounded by ~1e-4 there; reported, not the bar.
