@@ -91,6 +91,7 @@ struct fk_ctx {
   bool fuse_update = true;   // RK update inside pass F (else residual + rk_update)
   bool stage_f = true;       // pass F stages the workgroup's face states in LDS (mixed only)
   bool prims_w = true;       // mixed: producers of U also write wprim, pass G reads it
+  bool fuse_relax = true;    // surrogate relaxation stage 1 inside the last fused pass F
   // FP32 primitive records [C][NEQ] of U[ia] (mixed precision), valid while
   // w_valid: set by the kernels that write them with a new state, cleared by
   // every other change of U
@@ -104,6 +105,8 @@ struct fk_ctx {
 
 // defined in fk_mlp.hip
 int fk_relax_mlp_impl(fk_ctx *c, bool warm);
+int fk_relax_mlp_begin(fk_ctx *c, bool warm);   // checks, scratch, zeroed counts
+int fk_relax_mlp_finish(fk_ctx *c, bool warm);  // stage 2 (work list) after stage 1
 int ensure_rlist(fk_ctx *c);  // defined in fk_kernels.hip
 int fk_relax_emit_impl(fk_ctx *c, bool *done);
 void fk_mlp_free(fk_ctx *c);

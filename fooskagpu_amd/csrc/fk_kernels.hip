@@ -282,7 +282,13 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
 // A template flag rather than a null test: the FP64 conversion at the end
 // makes the kernel spill at 3 waves per SIMD (8.4 against 7.6 ms), so the
 // stages whose records are not read use the variant without it.
-template <int NEQ, typename T, bool VISC, bool HEAT, int MODE, bool STAGE, bool WOUT = false>
+//
+// RLX (MODE 0, two-phase): the last RK stage also runs stage 1 of the
+// surrogate relaxation (fk::screen_cell) on the state it has just formed, so
+// the state is not read back by a separate kernel.  Listed cells go to
+// rlist (length in rcnt[5]); each wave stores its [n_chem, iterations] in
+// rpart[wave]; Wout gets the primitive records of the cells solved here.
+template <int NEQ, typename T, bool VISC, bool HEAT, int MODE, bool STAGE, bool WOUT = false, bool RLX = false>
 __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3)) pass_f(fk_params P, int64_t nc, const double *__restrict__ U,
                                                  const double *__restrict__ U0, double *__restrict__ Unew,
                                                  const int32_t *__restrict__ link,
@@ -291,7 +297,10 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
                                                  const T *__restrict__ face, const T *__restrict__ gvel,
                                                  double *__restrict__ rhs_out, double *__restrict__ dtc, int irk,
                                                  double dt_val, const double *__restrict__ dt_ptr,
-                                                 float *__restrict__ Wout) {
+                                                 float *__restrict__ Wout, uint32_t *__restrict__ rlist,
+                                                 unsigned long long *__restrict__ rcnt,
+                                                 unsigned long long *__restrict__ rpart) {
+  static_assert(!RLX || (MODE == 0 && NEQ == 10), "fused relaxation: two-phase RK stage only");
   constexpr int NGS = ngrad<NEQ, HEAT>();
   const int64_t c0 = xcd_block() * kBlock;
   const int64_t c = c0 + threadIdx.x;
@@ -330,7 +339,16 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
     }
     __syncthreads();
   }
-  if (c >= nc) return;
+  if (c >= nc) {
+    // a wave with no cell in range (lane 0 holds its lowest cell) still
+    // stores its relaxation counts
+    if constexpr (RLX)
+      if ((threadIdx.x & 63) == 0) {
+        rpart[(c >> 6) * 2] = 0;
+        rpart[(c >> 6) * 2 + 1] = 0;
+      }
+    return;
+  }
   // face state of global slot `sl` of cell `cell`, from LDS when staged
   auto slot_ptr = [&](int64_t cell, int64_t sl) -> const T * {
     if (STAGE && cell >= c0 && cell < c0 + kBlock) return sF + (sl - c0 * 4) * NEQ;
@@ -557,14 +575,46 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
       if (P.rkSteps == 1 || irk == 0) u = (1 - neglect) * (base[v] + dt * (1.0 / Vc) * rhs[v]);
       else if (irk == 1) u = (1 - neglect) * (base[v] + (1.0 / 4.0) * dt * (1.0 / Vc) * rhs[v]);
       else u = (1 - neglect) * (base[v] + (2.0 / 3.0) * dt * (1.0 / Vc) * rhs[v]);
-      Unew[c * NEQ + v] = u;
+      if constexpr (!RLX) Unew[c * NEQ + v] = u;
       un[v] = u;
+    }
+    if constexpr (RLX) {
+      int chem = 0, it = 0;
+      const int push = fk::screen_cell(P, un, &chem, &it);
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) Unew[c * NEQ + v] = un[v];
+      if (Wout && !push) {
+        fk::Mat<double> mw = md;
+        asm volatile("" : "+s"(mw.g[0]), "+s"(mw.g[1]), "+s"(mw.pi[0]), "+s"(mw.pi[1]), "+s"(mw.eta[0]),
+                     "+s"(mw.eta[1]), "+s"(mw.avfMin));
+        fk::store_prims<NEQ>(mw, un, Wout + c * NEQ);
+      }
+      // wave-aggregated append to the work list (its order does not affect
+      // results), and the wave's counts without atomics: it <= 4, so three
+      // ballots give the iteration sum
+      const int l = threadIdx.x & 63;
+      const unsigned long long mp = __ballot(push);
+      if (mp) {
+        const int leader = __ffsll((long long)mp) - 1;
+        unsigned long long b = 0;
+        if (l == leader) b = atomicAdd(&rcnt[5], (unsigned long long)__popcll(mp));
+        b = __shfl(b, leader, 64);
+        if (push) rlist[b + __popcll(mp & ((1ull << l) - 1ull))] = (uint32_t)((c << 1) | chem);
+      }
+      static_assert(fk::kColdIters < 8, "iteration counts are summed from three ballots");
+      const unsigned long long nch = __popcll(__ballot(chem));
+      const unsigned long long nit = __popcll(__ballot(it & 1)) + 2ull * __popcll(__ballot(it & 2)) +
+                                     4ull * __popcll(__ballot(it & 4));
+      if (l == 0) {
+        rpart[(c >> 6) * 2] = nch;
+        rpart[(c >> 6) * 2 + 1] = nit;
+      }
     }
     // mixed precision: FP32 primitive record of the new state for pass G.  The
     // EOS constants are re-derived from the (scalar) parameters here: shared
     // with the cell pressure above, they would otherwise be held in vector
     // registers across the face loop and spill.
-    if constexpr (WOUT) {
+    if constexpr (WOUT && !RLX) {
       fk::Mat<double> mw = md;
       asm volatile("" : "+s"(mw.g[0]), "+s"(mw.g[1]), "+s"(mw.pi[0]), "+s"(mw.pi[1]), "+s"(mw.eta[0]), "+s"(mw.eta[1]),
                    "+s"(mw.avfMin));
@@ -891,7 +941,8 @@ int launch_pass_g_t(fk_ctx *c) {
 }
 
 template <typename T, int MODE>
-int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Unew, bool write_w = false) {
+int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Unew, bool write_w = false,
+                    bool relax = false) {
   const unsigned gr = grid_xcd(c->nc);
   const bool visc = c->p.viscFlag == 1;
   const bool heat = visc && c->p.model == FK_MODEL_SINGLE && c->p.heatFlux == 1;
@@ -901,33 +952,48 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
   const T *wl = (const T *)c->wls;
   const char *nm = MODE == 0 ? "pass_f" : (MODE == 1 ? "pass_f_rhs" : "pass_f_res");
   float *Wout = (MODE == 0 && sizeof(T) == 4 && write_w) ? c->wprim : nullptr;
+  // fused relaxation (last stage): records of the relaxed state, mixed precision
+  float *Wr = (sizeof(T) == 4 && c->prims_w) ? c->wprim : nullptr;
+  unsigned long long *cnt = (unsigned long long *)c->lcount;
   // LDS staging of face states (mixed precision only: 40 KB + 9 KB per workgroup)
   const bool stage = c->stage_f && sizeof(T) == 4;
   const size_t lds = stage ? (size_t)kBlock * (4 * c->neq + 12) * sizeof(T) : 0;
+#define PF_ARGS(W) \
+  c->p, c->nc, U, U0, Unew, c->link, c->geo, wl, c->vol, c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp, W
 #define PF(NEQ, VISC, HEAT)                                                                                            \
   do {                                                                                                                 \
+    if constexpr (MODE == 0 && NEQ == 10) {                                                                            \
+      if (relax) {                                                                                                     \
+        if (stage)                                                                                                     \
+          LAUNCH_TIMED(c, "pass_f_relax",                                                                              \
+                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, true, false, true>), dim3(gr), dim3(kBlock), \
+                                          lds, c->stream, PF_ARGS(Wr), c->rlist, cnt, c->rpart));                      \
+        else                                                                                                           \
+          LAUNCH_TIMED(c, "pass_f_relax",                                                                              \
+                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false, false, true>), dim3(gr),             \
+                                          dim3(kBlock), 0, c->stream, PF_ARGS(Wr), c->rlist, cnt, c->rpart));          \
+        break;                                                                                                         \
+      }                                                                                                                \
+    }                                                                                                                  \
     if constexpr (MODE == 0 && sizeof(T) == 4) {                                                                       \
       if (Wout) {                                                                                                      \
         if (stage)                                                                                                     \
           LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, true, true>), dim3(gr),             \
-                                                 dim3(kBlock), lds, c->stream, c->p, c->nc, U, U0, Unew, c->link,       \
-                                                 c->geo, wl, c->vol, c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp, \
-                                                 Wout));                                                                \
+                                                 dim3(kBlock), lds, c->stream, PF_ARGS(Wout), nullptr, nullptr,         \
+                                                 nullptr));                                                             \
         else                                                                                                           \
           LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false, true>), dim3(gr),            \
-                                                 dim3(kBlock), 0, c->stream, c->p, c->nc, U, U0, Unew, c->link, c->geo, \
-                                                 wl, c->vol, c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp, Wout)); \
+                                                 dim3(kBlock), 0, c->stream, PF_ARGS(Wout), nullptr, nullptr,           \
+                                                 nullptr));                                                             \
         break;                                                                                                         \
       }                                                                                                                \
     }                                                                                                                  \
     if (stage)                                                                                                         \
       LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, true>), dim3(gr), dim3(kBlock), lds,     \
-                                             c->stream, c->p, c->nc, U, U0, Unew, c->link, c->geo, wl, c->vol,          \
-                                             c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp, Wout));                 \
+                                             c->stream, PF_ARGS(Wout), nullptr, nullptr, nullptr));                     \
     else                                                                                                               \
       LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false>), dim3(gr), dim3(kBlock), 0,      \
-                                             c->stream, c->p, c->nc, U, U0, Unew, c->link, c->geo, wl, c->vol,          \
-                                             c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp, Wout));                 \
+                                             c->stream, PF_ARGS(Wout), nullptr, nullptr, nullptr));                     \
   } while (0)
   if (c->neq == 5) {
     if (heat) PF(5, true, true);
@@ -938,6 +1004,7 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
     else PF(10, false, false);
   }
 #undef PF
+#undef PF_ARGS
   return FK_OK;
 }
 
@@ -959,7 +1026,19 @@ int do_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want_w);
 // RK stage irk from pass F: fused (MODE 0) or residual (MODE 2) + rk_update,
 // as c->fuse_update selects; both give the same bits.  want_w: the fused
 // kernel also writes the primitive records of the new state (mixed precision).
-int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want_w) {
+// relax (fused form only): the kernel also runs stage 1 of the surrogate
+// relaxation (fk_relax_mlp_begin before, fk_relax_mlp_finish after).
+int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want_w, bool relax = false) {
+  if (relax) {
+    const int nb = next_buffer(c);
+    int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 0>(c, irk, dt, dtp, c->U[nb], false, true)
+                                        : launch_pass_f_t<double, 0>(c, irk, dt, dtp, c->U[nb], false, true);
+    if (rc) return rc;
+    c->ia = nb;
+    c->w_valid = false;  // until stage 2 has written the listed cells' records
+    c->rhs_valid = false;
+    return FK_OK;
+  }
   if (!c->fuse_update) {
     int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 2>(c, 0, 0.0, nullptr, nullptr)
                                         : launch_pass_f_t<double, 2>(c, 0, 0.0, nullptr, nullptr);
@@ -1045,7 +1124,9 @@ int ensure_rlist(fk_ctx *c) {
   c->rpart = nullptr;
   c->rlist_cap = 0;
   HIPCHK(hipMalloc(&c->rlist, (size_t)c->nc * sizeof(uint32_t)));
-  HIPCHK(hipMalloc(&c->rpart, (size_t)grid_for(c->nc) * 4 * sizeof(unsigned long long)));
+  // [blocks][4] partials of the relaxation kernels, or [waves][2] of the
+  // fused pass F (grid_xcd blocks of 4 waves)
+  HIPCHK(hipMalloc(&c->rpart, (size_t)grid_xcd(c->nc) * 8 * sizeof(unsigned long long)));
   c->rlist_cap = c->nc;
   return FK_OK;
 }
@@ -1091,6 +1172,21 @@ int do_relax(fk_ctx *c, fk_relax_stats *st, int mode = FK_RELAX_PLAIN) {
     HIPCHK(hipGetLastError());
     ++c->relax_calls;
   }
+  return FK_OK;
+}
+
+// After a fused pass F ran stage 1 of the surrogate relaxation: its per-wave
+// counts, stage 2 and the running totals (as do_relax).
+int relax_fused_finish(fk_ctx *c, bool warm) {
+  unsigned long long *cnt = (unsigned long long *)c->lcount;
+  const int64_t nw = (int64_t)grid_xcd(c->nc) * (kBlock / 64);
+  LAUNCH_TIMED(c, "relax_count", hipLaunchKernelGGL(fk::reduce_partials<2>, dim3(fk::reduce_grid(nw)), dim3(256), 0,
+                                                    c->stream, c->rpart, nw, cnt));
+  int rc = fk_relax_mlp_finish(c, warm);
+  if (rc) return rc;
+  hipLaunchKernelGGL(accum_counts, dim3(1), dim3(64), 0, c->stream, cnt);
+  HIPCHK(hipGetLastError());
+  ++c->relax_calls;
   return FK_OK;
 }
 
@@ -1388,6 +1484,10 @@ int fk_set_tuning(fk_ctx *c, const char *name, int64_t value) {
     c->w_valid = false;
     return FK_OK;
   }
+  if (std::strcmp(name, "fuse_relax") == 0) {
+    c->fuse_relax = value != 0;
+    return FK_OK;
+  }
   return fail(FK_EINVAL, std::string("unknown tuning option ") + name);
 }
 
@@ -1445,6 +1545,9 @@ int fk_run_steps(fk_ctx *c, int nsteps, double *dt_last, double *phys_time) {
     // the last stage's state goes to the relaxation, which writes the
     // primitive records itself
     const bool relax_next = p.model == FK_MODEL_TWO_PHASE && p.relaxFlag == 1;
+    const bool mlp_relax = relax_next && (p.relaxMode == FK_RELAX_MLP || p.relaxMode == FK_RELAX_COLD);
+    const bool warm = p.relaxMode == FK_RELAX_MLP;
+    bool relaxed_in_f = false;
     for (int irk = 0; irk < p.rkSteps; ++irk) {
       const bool want_w = irk < p.rkSteps - 1 || !relax_next;
       if ((rc = do_pass_g(c))) return rc;
@@ -1452,18 +1555,25 @@ int fk_run_steps(fk_ctx *c, int nsteps, double *dt_last, double *phys_time) {
         if ((rc = do_pass_f_rhs(c))) return rc;
         if ((rc = do_dt_device(c))) return rc;
         if ((rc = do_update(c, irk, 0.0, c->dscal, want_w))) return rc;
-      } else if (p.adaptDt == 1) {
-        if ((rc = do_pass_f_update(c, irk, 0.0, c->dscal, want_w))) return rc;
       } else {
-        if ((rc = do_pass_f_update(c, irk, p.dt, nullptr, want_w))) return rc;
+        // the last stage's fused pass F runs stage 1 of the surrogate relaxation
+        const bool fr = mlp_relax && c->fuse_relax && c->fuse_update && irk == p.rkSteps - 1;
+        if (fr && (rc = fk_relax_mlp_begin(c, warm))) return rc;
+        if (p.adaptDt == 1) rc = do_pass_f_update(c, irk, 0.0, c->dscal, want_w, fr);
+        else rc = do_pass_f_update(c, irk, p.dt, nullptr, want_w, fr);
+        if (rc) return rc;
+        relaxed_in_f = fr;
       }
     }
     if (p.adaptDt == 1)
       LAUNCH_TIMED(c, "advance_time", hipLaunchKernelGGL(advance_time, dim3(1), dim3(64), 0, c->stream, c->dscal));
     else
       c->dt = p.dt;
-    if (p.model == FK_MODEL_TWO_PHASE && p.relaxFlag == 1)
+    if (relaxed_in_f) {
+      if ((rc = relax_fused_finish(c, warm))) return rc;
+    } else if (relax_next) {
       if ((rc = do_relax(c, nullptr, p.relaxMode))) return rc;
+    }
     c->step++;
   }
   HIPCHK(hipMemcpyAsync(h, c->dscal, sizeof h, hipMemcpyDeviceToHost, c->stream));

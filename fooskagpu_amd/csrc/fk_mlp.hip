@@ -378,10 +378,11 @@ __global__ void __launch_bounds__(256) mlp_label_k(fk_params P, int64_t n, const
 // classification by the sign of f(T_L; p_L); thermal cells try at most
 // kColdIters analytic-Jacobian Newton iterations from the cell's own state
 // (one hydro step away from the previous equilibrium, so most converge in 2-3)
-// and are written back when converged.  Chemical cells and thermal cells that
-// did not converge go to the work list as (cell << 1) | chem for stage 2.
+// and are written back when converged (fk::screen_cell).  Chemical cells and
+// thermal cells that did not converge go to the work list as
+// (cell << 1) | chem for stage 2.  The last RK stage's fused pass F runs the
+// same stage 1 on the state it produces (fk_kernels.hip).
 // counts: [n_chem, newton iterations, -, fallbacks, warm-started, list length].
-constexpr int kColdIters = 4;
 
 __global__ void __launch_bounds__(256) relax_screen_k(fk_params P, int64_t nc, double *__restrict__ U,
                                                       uint32_t *__restrict__ list,
@@ -393,29 +394,11 @@ __global__ void __launch_bounds__(256) relax_screen_k(fk_params P, int64_t nc, d
   if (c < nc) {
     double u[10];
     fk::load_rec<10>(U + c * 10, u);
-    if (P.chemFlag == 1) {
-      const double avf = fk::clampd(P, u[0]);
-      const double rho = u[1] / avf, enr = u[6] / (avf * rho);
-      const double tL = fk::eos_T(P, 0, rho, enr), pL = fk::eos_p(P, 0, rho, enr);
-      chem = avf > P.chemLimit && fk::sat_superheated(P, pL, tL);
-    }
-    if (chem) {
-      push = 1;
-    } else {
-      fk::ThermProb t;
-      fk::therm_setup(P, u, t);
-      double xr, xe;
-      fk::therm_cold(P, u, t, xr, xe);
-      it = fk::therm_newton(P, t, xr, xe, P.polishTol, kColdIters);
-      if (it > 0) {
-        fk::therm_commit(P, t, xr, xe, u);
-        U[c * 10 + 0] = u[0];
-        U[c * 10 + 6] = u[6];
-        U[c * 10 + 7] = u[7];
-      } else {
-        it = kColdIters;
-        push = 1;
-      }
+    push = fk::screen_cell(P, u, &chem, &it);
+    if (!push) {
+      U[c * 10 + 0] = u[0];
+      U[c * 10 + 6] = u[6];
+      U[c * 10 + 7] = u[7];
     }
     // primitive records of pass G for the cells solved here (list cells:
     // relax_list_k)
@@ -623,7 +606,9 @@ int ensure_data(fk_ctx *c, int64_t cap) {
 
 // called from the relaxation entry point of fk_kernels.hip; counts layout as
 // relax_screen_k / relax_list_k (slots [0, 6) of c->lcount, zeroed here).
-int fk_relax_mlp_impl(fk_ctx *c, bool warm) {
+// Checks, scratch and zeroed counts of the surrogate relaxation (before stage
+// 1, which relax_screen_k or the last stage's fused pass F runs).
+int fk_relax_mlp_begin(fk_ctx *c, bool warm) {
   Mlp &m = c->mlp;
   if (warm && !m.ready) return fail(FK_ESTATE, "surrogate relaxation needs trained or loaded MLP weights");
   // the exact classification needs f(T) increasing: B <= 0 and C >= 0
@@ -634,6 +619,13 @@ int fk_relax_mlp_impl(fk_ctx *c, bool warm) {
   if (rc) return rc;
   unsigned long long *cnt = (unsigned long long *)c->lcount;
   LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_u64_k, dim3(1), dim3(64), 0, c->stream, cnt, 6));
+  return FK_OK;
+}
+
+int fk_relax_mlp_impl(fk_ctx *c, bool warm) {
+  int rc = fk_relax_mlp_begin(c, warm);
+  if (rc) return rc;
+  unsigned long long *cnt = (unsigned long long *)c->lcount;
   const unsigned gs = (unsigned)((c->nc + 255) / 256);
   // mixed precision: the relaxation also writes the primitive records of pass G
   float *W = (c->p.precision == FK_MIXED && c->prims_w) ? c->wprim : nullptr;
@@ -641,6 +633,14 @@ int fk_relax_mlp_impl(fk_ctx *c, bool warm) {
                                                      c->U[c->ia], c->rlist, cnt, c->rpart, W));
   LAUNCH_TIMED(c, "relax_count", hipLaunchKernelGGL(fk::reduce_partials<2>, dim3(fk::reduce_grid(gs)), dim3(256), 0,
                                                     c->stream, c->rpart, (int64_t)gs, cnt));
+  return fk_relax_mlp_finish(c, warm);
+}
+
+// Stage 2 (the work list) after stage 1.
+int fk_relax_mlp_finish(fk_ctx *c, bool warm) {
+  Mlp &m = c->mlp;
+  unsigned long long *cnt = (unsigned long long *)c->lcount;
+  float *W = (c->p.precision == FK_MIXED && c->prims_w) ? c->wprim : nullptr;
   const int64_t nchunk = (c->nc + 63) / 64;
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nchunk + 3) / 4, 1024));
   if (warm)

@@ -485,6 +485,41 @@ FK_HD inline int therm_polish(const fk_params &P, double *U, double rk0, double 
   return it;
 }
 
+// Stage 1 of the surrogate relaxation for one cell, in place on u: the exact
+// classification (chemical when alpha_L > chemLimit and the liquid is
+// superheated at its own pressure), and for a thermal cell at most
+// kColdIters analytic-Jacobian Newton iterations from the cell's own state,
+// committed when converged.  Returns 1 when the cell goes to the work list
+// of stage 2 (chemical, or not converged); *chem is the class and *iters the
+// iterations (kColdIters for a failed cold start, 0 for a chemical cell).
+constexpr int kColdIters = 4;
+FK_HD inline int screen_cell(const fk_params &P, double *u, int *chem, int *iters) {
+  int ch = 0;
+  if (P.chemFlag == 1) {
+    const double avf = clampd(P, u[0]);
+    const double rho = u[1] / avf, enr = u[6] / (avf * rho);
+    const double tL = eos_T(P, 0, rho, enr), pL = eos_p(P, 0, rho, enr);
+    ch = avf > P.chemLimit && sat_superheated(P, pL, tL);
+  }
+  *chem = ch;
+  if (ch) {
+    *iters = 0;
+    return 1;
+  }
+  ThermProb t;
+  therm_setup(P, u, t);
+  double xr, xe;
+  therm_cold(P, u, t, xr, xe);
+  const int it = therm_newton(P, t, xr, xe, P.polishTol, kColdIters);
+  if (it > 0) {
+    therm_commit(P, t, xr, xe, u);
+    *iters = it;
+    return 0;
+  }
+  *iters = kColdIters;
+  return 1;
+}
+
 // Chemical (p-T-g) equilibrium of chemRlx (chemRlx.cu:52-76, 195-215) by
 // Newton with a forward-difference Jacobian (step 1e-7 |x|) and full steps,
 // from the warm start (alpha_k, rho_k, e_k), to |dx_i| <= tol |x_i|.

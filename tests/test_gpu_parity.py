@@ -371,3 +371,30 @@ def test_finite_rate_refuses_the_surrogate_relaxation(gpu):
         with pytest.raises(FkError, match="finite-rate"):
             g.relaxation(_lib.RELAX_MLP)
         g.relaxation(_lib.RELAX_PLAIN)
+
+
+@pytest.mark.parametrize("precision", [_lib.FP64, _lib.MIXED])
+@pytest.mark.parametrize("mode", [_lib.RELAX_COLD, _lib.RELAX_MLP])
+def test_fused_relaxation_stage_equals_separate_kernel(gpu, precision, mode):
+    """fuse_relax: stage 1 of the surrogate relaxation inside the last RK
+    stage's fused pass F, or in its own kernel after it: bitwise equal states
+    and equal counts (cavitation: thermal and chemical cells; shock-bubble)."""
+    from fooskagpu_amd import surrogate
+    rng = np.random.default_rng(0)
+    x = (rng.normal(size=(1000, 3)) * [2.0, 3e5, 0.3] + [5.0, 1.4e6, 0.5]).astype(np.float32)
+    y = (rng.normal(size=(1000, 3)) * [20.0, 2.0, 0.3] + [300.0, 5.0, 0.5]).astype(np.float32)
+    for name in ("cavitation", "shock_bubble"):
+        c = SMALL[name]()
+        p = dict(c.params, precision=precision, relaxMode=mode)
+        out, tot = [], []
+        for fused in (0, 1):
+            with GpuSolver(p, c.mesh, c.U) as g:
+                if mode == _lib.RELAX_MLP:
+                    surrogate.Surrogate(g).set_weights(surrogate.init_weights(1), surrogate.fit_norm(x, y))
+                g.set_tuning("fuse_relax", fused)
+                g.run_steps(STEPS.get(name, 3))
+                out.append(g.download_state())
+                tot.append(g.relax_totals())
+        print(name, tot[1], flush=True)
+        assert np.isfinite(out[0]).all()
+        assert np.array_equal(out[0], out[1]) and tot[0] == tot[1]
