@@ -1536,6 +1536,9 @@ int fk_run_steps(fk_ctx *c, int nsteps, double *dt_last, double *phys_time) {
   if (nsteps < 0) return fail(FK_EINVAL, "negative step count");
   HIPCHK(hipSetDevice(c->device));
   const fk_params &p = c->p;
+  // a surrogate relaxation without weights fails here, before any stage ran
+  if (p.model == FK_MODEL_TWO_PHASE && p.relaxFlag == 1 && p.relaxMode == FK_RELAX_MLP && !c->mlp.ready && nsteps > 0)
+    return fail(FK_ESTATE, "surrogate relaxation needs trained or loaded MLP weights");
   // the device scalars hold dt and the physical time
   double h[2] = {c->dt, c->time};
   HIPCHK(hipMemcpy(c->dscal, h, sizeof h, hipMemcpyHostToDevice));

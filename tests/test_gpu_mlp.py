@@ -165,3 +165,12 @@ def at_equilibrium(U, p):
     T = (e - eta - pi / r) * g / cp
     return (np.abs(pk[:, 0] - pk[:, 1]) <= 1e-6 * np.maximum(np.abs(pk[:, 0]), 1e3)) & \
         (np.abs(T[:, 0] - T[:, 1]) <= 1e-3)
+
+
+def test_surrogate_steps_without_weights_fail_before_any_stage(gpu):
+    from fooskagpu_amd._lib import FkError
+    c = cases.bubble(n=8)
+    with GpuSolver(dict(c.params, relaxMode=_lib.RELAX_MLP), c.mesh, c.U) as g:
+        with pytest.raises(FkError, match="weights"):
+            g.run_steps(2)
+        assert np.array_equal(g.download_state(), c.U) and g.clock()[2] == 0
