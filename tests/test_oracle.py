@@ -183,6 +183,37 @@ def test_time_step_single_tet(tmp_path):
     assert dt == pytest.approx(0.5 * m.volume[0] ** 0.3333 / smax, rel=1e-13)
 
 
+@pytest.mark.parametrize("rho_yv,woods", [(1.0, 1), (0.0, 1), (1.0, 0)])
+def test_two_phase_time_step_uses_wood_or_frozen_sound_speed(tmp_path, rho_yv, woods):
+    """Single tet at rest, alpha_L = 0.5: the signal speed is the mixture sound
+    speed, Wood's 1/(rho c^2) = sum_k alpha_k / (rho_k c_k^2) where
+    rho Y_v > chemLimit and woodsFlag == 1, else the frozen
+    c^2 = sum_k Y_k c_k^2, with c_k^2 = gamma_k (p_k + pinf_k) / rho_k."""
+    path = tmp_path / "tet.neu"
+    path.write_text(
+        "CONTROL INFO 1.2.1\nNUMNP NELEM NGRPS NBSETS NDFCD NDFVL\n 4 1 1 1 3 3\nENDOFSECTION\n"
+        "NODAL COORDINATES 1.2.1\n1 0 0 0\n2 1 0 0\n3 0 1 0\n4 0 0 1\nENDOFSECTION\n"
+        "ELEMENTS/CELLS 1.2.1\n1 6 4 1 2 3 4\nENDOFSECTION\n"
+        "BOUNDARY CONDITIONS 1.2.1\nout 1 4 0 26\n1 6 1\n1 6 2\n1 6 3\n1 6 4\nENDOFSECTION\n")
+    m = Mesh.read_neu(path, rcm=False)
+    prm = dict(TP, woodsFlag=woods)
+    a, p = 0.5, 2e5
+    rL, rG = po.eqos(TP, 0, 2, p, 300.0), po.eqos(TP, 1, 2, p, 300.0)
+    W = np.array([a, rL, rG, 0.0, 0.0, 0.0, p, p, rho_yv, 0.0])
+    o = Oracle(prm, m, po.prim2cons(prm, W)[None])
+    o.compute_face_values()
+    o.compute_rhs()
+    dt = o.compute_time_step()
+    g, pi = TP["gamma"], TP["pInf"]
+    c2 = [g[k] * (p + pi[k]) / r for k, r in ((0, rL), (1, rG))]
+    rho = a * rL + (1 - a) * rG
+    if rho_yv > TP["chemLimit"] and woods == 1:
+        c = 1 / math.sqrt(rho * (a / (rL * c2[0]) + (1 - a) / (rG * c2[1])))
+    else:
+        c = math.sqrt((a * rL * c2[0] + (1 - a) * rG * c2[1]) / rho)
+    assert dt == pytest.approx(TP["cfl"] * m.volume[0] ** 0.3333 / c, rel=1e-12)
+
+
 # ---------------- whole-step invariants ----------------
 def periodic_box(n=4, jitter=0.1):
     return Mesh.box(n, n, n, (0, 0, 0), (1, 1, 1), jitter, 11, periodic=(True, True, True))
