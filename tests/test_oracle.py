@@ -333,6 +333,14 @@ def test_limiter_bounds_and_uniform_field():
     lo = np.minimum(0, (wmin - W) / 2)[:, None, :]
     hi = np.maximum(0, (wmax - W) / 2)[:, None, :]
     assert np.all(d >= lo - 1e-12) and np.all(d <= hi + 1e-12)
+    # the bound is reached: where 0 < theta < 1, some face of the cell sits on
+    # W + (wmax - W) / 2 or W + (wmin - W) / 2 (a factor other than 1/2 in the
+    # limiter misses it)
+    scale = np.abs(W)[:, None, :] + 1e-300
+    on_bound = (np.abs(d - lo) <= 1e-11 * scale) | (np.abs(d - hi) <= 1e-11 * scale)
+    active = (th > 1e-9) & (th < 1 - 1e-9)
+    assert active.sum() > 20
+    assert np.all(on_bound.any(axis=1)[active])
     # uniform field: zero gradients, face states equal the cell state
     o2 = Oracle(c.params, m, np.tile(c.U[0], (m.n_cells, 1)))
     o2.compute_face_values()
