@@ -219,6 +219,49 @@ def periodic_box(n=4, jitter=0.1):
     return Mesh.box(n, n, n, (0, 0, 0), (1, 1, 1), jitter, 11, periodic=(True, True, True))
 
 
+@pytest.mark.parametrize("rk", [1, 3])
+def test_interface_advection_keeps_pressure_and_velocity_uniform(rk):
+    """A smooth two-phase interface (alpha_L from 0.1 to 0.9) advected at
+    uniform u = 10 m/s and p = 1e5 Pa, equal phase temperatures, periodic
+    jittered box, no relaxation: alpha moves while u and both phase pressures
+    stay uniform to round-off.  This pins the consistency of the phase mass
+    and energy fluxes with the volume-fraction transport; the
+    non-conservative terms vanish at uniform u and p."""
+    prm = dict(TP, relaxFlag=0, chemFlag=0, adaptDt=0, dt=2e-6, rkSteps=rk)
+    m = Mesh.box(8, 3, 3, (0, 0, 0), (1, 0.375, 0.375), 0.1, 5, periodic=(True, True, True))
+    p, u = 1e5, 10.0
+    aL = 0.1 + 0.8 * np.exp(-((m.cell_center[:, 0] - 0.5) / 0.15) ** 2)
+    rL, rG = po.eqos(prm, 0, 2, p, 300.0), po.eqos(prm, 1, 2, p, 300.0)
+    U = np.array([po.prim2cons(prm, np.array([a, rL, rG, u, 0, 0, p, p, 0.0, 0.0])) for a in aL])
+    o = Oracle(prm, m, U)
+    o.run_steps(3)
+    W = np.array([po.cons2prim(prm, x) for x in o.state()])
+    assert np.abs(W[:, 0] - aL).max() > 1e-4  # the interface moved
+    assert np.abs(W[:, 3] - u).max() < 1e-10 and np.abs(W[:, 4:6]).max() < 1e-10
+    assert np.abs(W[:, 7] - p).max() < 1e-9 * p
+    assert np.abs(W[:, 6] - p).max() < 1e-12 * (p + prm["pInf"][0])
+
+
+@pytest.mark.parametrize("rk", [1, 3])
+def test_compression_keeps_uniform_volume_fraction(rk):
+    """Uniform alpha_L = 0.5, p and T, velocity u = 10 sin(2 pi x): the
+    volume-fraction equation d alpha/dt + div(alpha u) - alpha div u = 0
+    leaves a uniform alpha unchanged while the densities change.  A missing
+    or mis-signed non-conservative term alpha (u . n) changes alpha by about
+    alpha div(u) dt."""
+    prm = dict(TP, relaxFlag=0, chemFlag=0, adaptDt=0, dt=2e-6, rkSteps=rk)
+    m = Mesh.box(8, 3, 3, (0, 0, 0), (1, 0.375, 0.375), 0.1, 5, periodic=(True, True, True))
+    p = 1e5
+    rL, rG = po.eqos(prm, 0, 2, p, 300.0), po.eqos(prm, 1, 2, p, 300.0)
+    ux = 10 * np.sin(2 * np.pi * m.cell_center[:, 0])
+    U = np.array([po.prim2cons(prm, np.array([0.5, rL, rG, v, 0, 0, p, p, 0.0, 0.0])) for v in ux])
+    o = Oracle(prm, m, U)
+    o.run_steps(3)
+    U2 = o.state()
+    assert np.abs(U2[:, 2] / U[:, 2] - 1).max() > 1e-6  # the gas density changed
+    assert np.abs(U2[:, 0] - 0.5).max() < 1e-13
+
+
 def test_free_stream_preserved_on_jittered_tets():
     m = periodic_box()
     W = np.array([1.1, 40.0, -10.0, 25.0, 1e5])
