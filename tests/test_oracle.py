@@ -262,6 +262,38 @@ def test_compression_keeps_uniform_volume_fraction(rk):
     assert np.abs(U2[:, 0] - 0.5).max() < 1e-13
 
 
+def test_two_phase_pure_liquid_limit_matches_single_phase_stiffened_gas():
+    """At alpha_L = 1 - avfMin/100 the two-phase model is the single-phase
+    stiffened-gas Euler system of the liquid: a 2 MPa pressure pulse in
+    water (periodic box, 10 SSP-RK3 steps, no relaxation, frozen sound speed)
+    evolves the same in both models up to the vapour's 1e-6 volume fraction.
+    Observed differences are 3.5e-5 (momentum) and 4.2e-5 (pressure
+    perturbation); an oracle without the face-pressure momentum term differs
+    by 1.0 and 0.11, one without the liquid energy term alpha_L p_L (u . n) by
+    2.8e-3 and 2.9e-3."""
+    base = dict(relaxFlag=0, chemFlag=0, adaptDt=0, dt=2e-6, rkSteps=3, woodsFlag=0)
+    tp = dict(TP, eta=[0.0, TP["eta"][1]], **base)
+    sp = dict(cases.BASE, model=0, gamma=[TP["gamma"][0]] * 2, pInf=[TP["pInf"][0]] * 2, Cp=[TP["Cp"][0]] * 2,
+              eta=[0.0, 0.0], **base)
+    m = Mesh.box(16, 3, 3, (0, 0, 0), (1, 0.1875, 0.1875), 0.1, 5, periodic=(True, True, True))
+    p = 1e7 + 2e6 * np.exp(-((m.cell_center[:, 0] - 0.5) / 0.1) ** 2)
+    rl = np.array([po.eqos(tp, 0, 2, q, 300.0) for q in p])
+    rg = np.array([po.eqos(tp, 1, 2, q, 300.0) for q in p])
+    am = 1 - tp["avfMin"] / 100
+    U2 = np.array([po.prim2cons(tp, np.array([am, a, b, 0, 0, 0, q, q, 0.0, 0.0])) for a, b, q in zip(rl, rg, p)])
+    U1 = np.array([po.prim2cons(sp, np.array([a, 0, 0, 0, q])) for a, q in zip(rl, p)])
+    o2, o1 = Oracle(tp, m, U2), Oracle(sp, m, U1)
+    o2.run_steps(10)
+    o1.run_steps(10)
+    A, B = o2.state(), o1.state()
+    W2 = np.array([po.cons2prim(tp, u) for u in A])
+    W1 = np.array([po.cons2prim(sp, u) for u in B])
+    assert np.abs(W1[:, 4] - p).max() > 1e5  # the pulse spread
+    assert np.abs((A[:, 1] + A[:, 2]) - B[:, 0]).max() / B[:, 0].max() < 1e-5
+    assert np.abs(A[:, 3] - B[:, 1]).max() / np.abs(B[:, 1]).max() < 3e-4
+    assert np.abs(W2[:, 6] - W1[:, 4]).max() / (W1[:, 4].max() - 1e7) < 3e-4
+
+
 def test_free_stream_preserved_on_jittered_tets():
     m = periodic_box()
     W = np.array([1.1, 40.0, -10.0, 25.0, 1e5])
