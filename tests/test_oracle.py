@@ -294,6 +294,35 @@ def test_two_phase_pure_liquid_limit_matches_single_phase_stiffened_gas():
     assert np.abs(W2[:, 6] - W1[:, 4]).max() / (W1[:, 4].max() - 1e7) < 3e-4
 
 
+def test_two_phase_pure_vapour_limit_matches_single_phase_ideal_gas():
+    """The vapour-side counterpart of the pure-liquid limit: alpha_L = 1e-10
+    (avfMin 1e-8), a 20 kPa pulse in vapour at 400 K against the single-phase
+    ideal gas (gamma 1.327).  Observed differences are 7e-4 (momentum) and
+    1.1e-3 (pressure perturbation); an oracle without the gas energy term
+    alpha_G p_G (u . n) differs by 0.16 and 0.21."""
+    base = dict(relaxFlag=0, chemFlag=0, adaptDt=0, dt=2e-5, rkSteps=3, woodsFlag=0, avfMin=1e-8)
+    tp = dict(TP, eta=[TP["eta"][0], 0.0], **base)
+    sp = dict(cases.BASE, model=0, gamma=[TP["gamma"][1]] * 2, pInf=[0.0, 0.0], Cp=[TP["Cp"][1]] * 2,
+              eta=[0.0, 0.0], **base)
+    m = Mesh.box(16, 3, 3, (0, 0, 0), (1, 0.1875, 0.1875), 0.1, 5, periodic=(True, True, True))
+    p = 1e5 + 2e4 * np.exp(-((m.cell_center[:, 0] - 0.5) / 0.1) ** 2)
+    rl = np.array([po.eqos(tp, 0, 2, q, 400.0) for q in p])
+    rg = np.array([po.eqos(tp, 1, 2, q, 400.0) for q in p])
+    am = tp["avfMin"] / 100
+    U2 = np.array([po.prim2cons(tp, np.array([am, a, b, 0, 0, 0, q, q, 0.0, 0.0])) for a, b, q in zip(rl, rg, p)])
+    U1 = np.array([po.prim2cons(sp, np.array([b, 0, 0, 0, q])) for b, q in zip(rg, p)])
+    o2, o1 = Oracle(tp, m, U2), Oracle(sp, m, U1)
+    o2.run_steps(10)
+    o1.run_steps(10)
+    A, B = o2.state(), o1.state()
+    W2 = np.array([po.cons2prim(tp, u) for u in A])
+    W1 = np.array([po.cons2prim(sp, u) for u in B])
+    assert np.abs(W1[:, 4] - p).max() > 5e3  # the pulse spread
+    assert np.abs((A[:, 1] + A[:, 2]) - B[:, 0]).max() / B[:, 0].max() < 1e-3
+    assert np.abs(A[:, 3] - B[:, 1]).max() / np.abs(B[:, 1]).max() < 5e-3
+    assert np.abs(W2[:, 7] - W1[:, 4]).max() / (W1[:, 4].max() - 1e5) < 5e-3
+
+
 def test_free_stream_preserved_on_jittered_tets():
     m = periodic_box()
     W = np.array([1.1, 40.0, -10.0, 25.0, 1e5])
