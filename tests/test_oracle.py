@@ -270,7 +270,8 @@ def test_two_phase_pure_liquid_limit_matches_single_phase_stiffened_gas():
     Observed differences are 3.5e-5 (momentum) and 4.2e-5 (pressure
     perturbation); an oracle without the face-pressure momentum term differs
     by 1.0 and 0.11, one without the liquid energy term alpha_L p_L (u . n) by
-    2.8e-3 and 2.9e-3."""
+    2.8e-3 and 2.9e-3.  Without the HLLC energy term the change of total
+    energy differs by 196 %."""
     base = dict(relaxFlag=0, chemFlag=0, adaptDt=0, dt=2e-6, rkSteps=3, woodsFlag=0)
     tp = dict(TP, eta=[0.0, TP["eta"][1]], **base)
     sp = dict(cases.BASE, model=0, gamma=[TP["gamma"][0]] * 2, pInf=[TP["pInf"][0]] * 2, Cp=[TP["Cp"][0]] * 2,
@@ -292,6 +293,15 @@ def test_two_phase_pure_liquid_limit_matches_single_phase_stiffened_gas():
     assert np.abs((A[:, 1] + A[:, 2]) - B[:, 0]).max() / B[:, 0].max() < 1e-5
     assert np.abs(A[:, 3] - B[:, 1]).max() / np.abs(B[:, 1]).max() < 3e-4
     assert np.abs(W2[:, 6] - W1[:, 4]).max() / (W1[:, 4].max() - 1e7) < 3e-4
+    # total energy: with Toro's star-state energy (hllcEnergy 1) the change
+    # of E matches the single-phase HLLC (observed 2.1e-4); the reference's
+    # form (hllcEnergy 0, no rho_K in the energy term) differs by 5.1 %
+    dE1 = B[:, 4] - U1[:, 4]
+    for h, lo, hi in ((0, 1e-2, 0.2), (1, 0.0, 1e-3)):
+        o3 = Oracle(dict(tp, hllcEnergy=h), m, U2)
+        o3.run_steps(10)
+        d = np.abs((o3.state()[:, 8] - U2[:, 8]) - dE1).max() / np.abs(dE1).max()
+        assert lo <= d < hi, (h, d)
 
 
 def test_two_phase_pure_vapour_limit_matches_single_phase_ideal_gas():
