@@ -59,7 +59,7 @@ typedef struct fk_params {
   int32_t thermItrMax, chemItrMax;
   int32_t hllcEnergy; /* 0: energy star state as in starRegion.h:32, 1: Toro's form */
   int32_t relaxFlag;  /* two-phase: run the per-step phase relaxation */
-  int32_t relaxMode;  /* FK_RELAX_PLAIN or FK_RELAX_MLP for the relaxation of fk_run_steps */
+  int32_t relaxMode;  /* FK_RELAX_PLAIN, FK_RELAX_MLP or FK_RELAX_COLD for the relaxation of fk_run_steps */
   double dt;          /* fixed time step (adaptDt == 0) */
   double cfl, maxDt, avfMin;
   double thermUrf, chemUrf, chemLimit, satTempUrf;
@@ -145,7 +145,9 @@ int fk_relax_totals(fk_ctx *ctx, fk_relax_stats *stats, int reset);
 int fk_cons_calc(fk_ctx *ctx, double *out);
 
 /* Whole time steps in explicitLoop order (dt is taken before the stage-0
- * update of every adaptDtItr-th step).  Returns the last dt and the time. */
+ * update of every adaptDtItr-th step), with the relaxation of relaxMode
+ * after the stages.  Returns the last dt and the time.  FK_RELAX_MLP
+ * without weights fails with FK_ESTATE before any stage runs. */
 int fk_run_steps(fk_ctx *ctx, int nsteps, double *dt_last, double *phys_time);
 
 /* Block until the context's stream is idle. */
