@@ -434,3 +434,45 @@ def test_all_boundary_conditions_match_oracle(gpu, model):
     print(f"[bcs model {model}] FP64 rel L1 {err.max():.2e}; mixed density {rho_err:.2e}", flush=True)
     assert np.isfinite(Ug).all() and err.max() <= FP64_TOL
     assert rho_err <= (MIXED_RHO_TOL if model == _lib.MODEL_TWO_PHASE else MIXED_RHO_TOL_SINGLE)
+
+
+def test_neglect_flags_match_oracle(gpu):
+    """neglectZ zeroes the z momentum in every RK update (update.cu:26-32);
+    neglectX and neglectY are inert there, because the reference assigns its
+    neglect flag three times and only the z test survives.  FP64 against the
+    oracle, fused and split RK updates."""
+    c = SMALL["shock_bubble"]()
+    p = dict(c.params, neglectX=1, neglectY=1, neglectZ=1)
+    o = Oracle(p, c.mesh, c.U)
+    o.run_steps(5)
+    Uo = o.state()
+    for fuse in (1, 0):
+        with GpuSolver(p, c.mesh, c.U) as g:
+            g.set_tuning("fuse_update", fuse)
+            g.run_steps(5)
+            Ug = g.download_state()
+        assert np.all(Ug[:, 5] == 0.0) and np.abs(Ug[:, 3]).max() > 0 and np.abs(Ug[:, 4]).max() > 0
+        assert rel_l1(Ug, Uo).max() <= FP64_TOL
+
+
+@pytest.mark.parametrize("flags", [dict(adaptDt=0, dt=1.5e-6), dict(rkSteps=1), dict(hllcEnergy=1),
+                                   dict(woodsFlag=0), dict(adaptDtItr=3)],
+                         ids=["fixed_dt", "euler", "toro_energy", "frozen_sound_speed", "dt_every_3"])
+def test_step_control_flags_match_oracle(gpu, flags):
+    """The INPUT switches of the step beyond the defaults: FP64 against the
+    oracle after 6 steps, mixed within its density bar (shock-bubble, WALE)."""
+    c = SMALL["shock_bubble"]()
+    p = dict(c.params, **flags)
+    o = Oracle(p, c.mesh, c.U)
+    dto, to = o.run_steps(6)
+    Uo = o.state()
+    with GpuSolver(p, c.mesh, c.U) as g:
+        dtg, tg = g.run_steps(6)
+        Ug = g.download_state()
+    with GpuSolver(dict(p, precision=_lib.MIXED), c.mesh, c.U) as g:
+        g.run_steps(6)
+        Um = g.download_state()
+    rho_err = rel_l1(Um[:, [1, 2]].sum(1, keepdims=True), Uo[:, [1, 2]].sum(1, keepdims=True))[0]
+    assert np.isfinite(Ug).all() and rel_l1(Ug, Uo).max() <= FP64_TOL
+    assert dtg == dto and tg == pytest.approx(to, rel=1e-14)
+    assert rho_err <= MIXED_RHO_TOL
