@@ -136,7 +136,7 @@ int match_neighbours(fkm_mesh &m) {
 // pseudo-peripheral cell found by repeated BFS (George-Liu), neighbours
 // queued by increasing degree (ties by index).  Returns perm[new] = old.
 std::vector<int64_t> rcm_order(const fkm_mesh &m) {
-  const int64_t nc = m.nc;
+  const int64_t nc = m.nc > 0 ? m.nc : 0;
   std::vector<int8_t> deg((size_t)nc, 0);
   for (int64_t c = 0; c < nc; ++c)
     for (int j = 0; j < 4; ++j) deg[(size_t)c] += (m.nb[(size_t)(c * 4 + j)] >= 0);
