@@ -220,7 +220,7 @@ def main():
                                + ("MLP warm start + FP64 Newton polish" if head == "mixed_mlp" else
                                   "plain Newton relaxation"),
                    "cells": C, "faces": case.mesh.n_faces, "cell_order": args.order, "fuse_update": args.fuse,
-                   "stage_f": args.stage_f},
+                   "stage_f": args.stage_f, "fuse_relax": 1},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": dom, "bytes_per_cell": bpc, "avg_launch_ms": avg_f * 1e3},
