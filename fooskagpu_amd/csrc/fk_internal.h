@@ -91,7 +91,7 @@ struct fk_ctx {
   bool fuse_update = true;   // RK update inside pass F (else residual + rk_update)
   bool stage_f = true;       // pass F stages the workgroup's face states in LDS (mixed only)
   bool prims_w = true;       // mixed: producers of U also write wprim, pass G reads it
-  bool fuse_relax = true;    // surrogate relaxation stage 1 inside the last fused pass F
+  bool fuse_relax = true;    // per-cell relaxation pass inside the last fused pass F
   // FP32 primitive records [C][NEQ] of U[ia] (mixed precision), valid while
   // w_valid: set by the kernels that write them with a new state, cleared by
   // every other change of U

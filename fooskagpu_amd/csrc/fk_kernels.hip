@@ -283,12 +283,14 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
 // makes the kernel spill at 3 waves per SIMD (8.4 against 7.6 ms), so the
 // stages whose records are not read use the variant without it.
 //
-// RLX (MODE 0, two-phase): the last RK stage also runs stage 1 of the
-// surrogate relaxation (fk::screen_cell) on the state it has just formed, so
-// the state is not read back by a separate kernel.  Listed cells go to
-// rlist (length in rcnt[5]); each wave stores its [n_chem, iterations] in
-// rpart[wave]; Wout gets the primitive records of the cells solved here.
-template <int NEQ, typename T, bool VISC, bool HEAT, int MODE, bool STAGE, bool WOUT = false, bool RLX = false>
+// RLX (MODE 0, two-phase): the last RK stage also relaxes the state it has
+// just formed, so the state is not read back by a separate kernel: 1 runs
+// stage 1 of the surrogate relaxation (fk::screen_cell), 2 the plain
+// per-cell relaxation (fk::plain_cell).  Cells left for a second kernel go to
+// rlist (length in rcnt[5]); each wave stores its counts in rpart[wave]
+// ([n_chem, iterations], and Tsat iterations for 2); Wout gets the primitive
+// records of the cells relaxed here.
+template <int NEQ, typename T, bool VISC, bool HEAT, int MODE, bool STAGE, bool WOUT = false, int RLX = 0>
 __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3)) pass_f(fk_params P, int64_t nc, const double *__restrict__ U,
                                                  const double *__restrict__ U0, double *__restrict__ Unew,
                                                  const int32_t *__restrict__ link,
@@ -301,6 +303,7 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
                                                  unsigned long long *__restrict__ rcnt,
                                                  unsigned long long *__restrict__ rpart) {
   static_assert(!RLX || (MODE == 0 && NEQ == 10), "fused relaxation: two-phase RK stage only");
+  constexpr int KR = RLX == 2 ? 3 : 2;  // counts per wave in rpart
   constexpr int NGS = ngrad<NEQ, HEAT>();
   const int64_t c0 = xcd_block() * kBlock;
   const int64_t c = c0 + threadIdx.x;
@@ -342,11 +345,9 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   if (c >= nc) {
     // a wave with no cell in range (lane 0 holds its lowest cell) still
     // stores its relaxation counts
-    if constexpr (RLX)
-      if ((threadIdx.x & 63) == 0) {
-        rpart[(c >> 6) * 2] = 0;
-        rpart[(c >> 6) * 2 + 1] = 0;
-      }
+    if constexpr (RLX != 0)
+      if ((threadIdx.x & 63) == 0)
+        for (int k = 0; k < KR; ++k) rpart[(c >> 6) * KR + k] = 0;
     return;
   }
   // face state of global slot `sl` of cell `cell`, from LDS when staged
@@ -575,10 +576,41 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
       if (P.rkSteps == 1 || irk == 0) u = (1 - neglect) * (base[v] + dt * (1.0 / Vc) * rhs[v]);
       else if (irk == 1) u = (1 - neglect) * (base[v] + (1.0 / 4.0) * dt * (1.0 / Vc) * rhs[v]);
       else u = (1 - neglect) * (base[v] + (2.0 / 3.0) * dt * (1.0 / Vc) * rhs[v]);
-      if constexpr (!RLX) Unew[c * NEQ + v] = u;
+      if constexpr (RLX == 0) Unew[c * NEQ + v] = u;
       un[v] = u;
     }
-    if constexpr (RLX) {
+    if constexpr (RLX == 2) {
+      int it = 0, ts = 0;
+      const int chem = fk::plain_cell(P, un, &it, &ts);
+#pragma unroll
+      for (int v = 0; v < NEQ; ++v) Unew[c * NEQ + v] = un[v];
+      if (Wout && !chem) {
+        fk::Mat<double> mw = md;
+        asm volatile("" : "+s"(mw.g[0]), "+s"(mw.g[1]), "+s"(mw.pi[0]), "+s"(mw.pi[1]), "+s"(mw.eta[0]),
+                     "+s"(mw.eta[1]), "+s"(mw.avfMin));
+        fk::store_prims<NEQ>(mw, un, Wout + c * NEQ);
+      }
+      // chemical cells to the list of relax_chem, wave-aggregated; counts
+      // summed bit by bit from ballots (no atomics on the counters)
+      const int l = threadIdx.x & 63;
+      const unsigned long long mp = __ballot(chem);
+      if (mp) {
+        const int leader = __ffsll((long long)mp) - 1;
+        unsigned long long b = 0;
+        if (l == leader) b = atomicAdd(&rcnt[5], (unsigned long long)__popcll(mp));
+        b = __shfl(b, leader, 64);
+        if (chem) rlist[b + __popcll(mp & ((1ull << l) - 1ull))] = (uint32_t)c;
+      }
+      unsigned long long nit = 0, nts = 0;
+      for (int k = 0; __ballot((it >> k) != 0); ++k) nit += (unsigned long long)__popcll(__ballot((it >> k) & 1)) << k;
+      for (int k = 0; __ballot((ts >> k) != 0); ++k) nts += (unsigned long long)__popcll(__ballot((ts >> k) & 1)) << k;
+      if (l == 0) {
+        rpart[(c >> 6) * 3] = (unsigned long long)__popcll(mp);
+        rpart[(c >> 6) * 3 + 1] = nit;
+        rpart[(c >> 6) * 3 + 2] = nts;
+      }
+    }
+    if constexpr (RLX == 1) {
       int chem = 0, it = 0;
       const int push = fk::screen_cell(P, un, &chem, &it);
 #pragma unroll
@@ -614,7 +646,7 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
     // EOS constants are re-derived from the (scalar) parameters here: shared
     // with the cell pressure above, they would otherwise be held in vector
     // registers across the face loop and spill.
-    if constexpr (WOUT && !RLX) {
+    if constexpr (WOUT && RLX == 0) {
       fk::Mat<double> mw = md;
       asm volatile("" : "+s"(mw.g[0]), "+s"(mw.g[1]), "+s"(mw.pi[0]), "+s"(mw.pi[1]), "+s"(mw.eta[0]), "+s"(mw.eta[1]),
                    "+s"(mw.avfMin));
@@ -743,17 +775,8 @@ __global__ void __launch_bounds__(kBlock) relax_therm(fk_params P, int64_t nc, d
   if (c < nc) {
     double u[10];
     fk::load_rec<10>(U + c * 10, u);
-    if (P.chemFlag == 1) {
-      const double avf = clampd(P, u[0]);
-      const double rho = u[1] / avf;
-      const double enr = u[6] / (avf * rho);
-      const double tL = eos_T(P, 0, rho, enr);
-      const double pL = eos_p(P, 0, rho, enr);
-      const double Ts = tsat_dev(P, pL, tL, P.satTempUrf, &ts);
-      chem = (tL > Ts && avf > P.chemLimit) ? 1 : 0;
-    }
+    chem = fk::plain_cell(P, u, &it, &ts);
     if (!chem) {
-      it = therm_dev(P, u);
       U[c * 10 + 0] = u[0];
       U[c * 10 + 6] = u[6];
       U[c * 10 + 7] = u[7];
@@ -942,7 +965,7 @@ int launch_pass_g_t(fk_ctx *c) {
 
 template <typename T, int MODE>
 int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Unew, bool write_w = false,
-                    bool relax = false) {
+                    int relax = 0) {
   const unsigned gr = grid_xcd(c->nc);
   const bool visc = c->p.viscFlag == 1;
   const bool heat = visc && c->p.model == FK_MODEL_SINGLE && c->p.heatFlux == 1;
@@ -963,15 +986,26 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
 #define PF(NEQ, VISC, HEAT)                                                                                            \
   do {                                                                                                                 \
     if constexpr (MODE == 0 && NEQ == 10) {                                                                            \
-      if (relax) {                                                                                                     \
+      if (relax == 1) {                                                                                                \
         if (stage)                                                                                                     \
           LAUNCH_TIMED(c, "pass_f_relax",                                                                              \
-                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, true, false, true>), dim3(gr), dim3(kBlock), \
+                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, true, false, 1>), dim3(gr), dim3(kBlock),   \
                                           lds, c->stream, PF_ARGS(Wr), c->rlist, cnt, c->rpart));                      \
         else                                                                                                           \
           LAUNCH_TIMED(c, "pass_f_relax",                                                                              \
-                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false, false, true>), dim3(gr),             \
-                                          dim3(kBlock), 0, c->stream, PF_ARGS(Wr), c->rlist, cnt, c->rpart));          \
+                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false, false, 1>), dim3(gr), dim3(kBlock),  \
+                                          0, c->stream, PF_ARGS(Wr), c->rlist, cnt, c->rpart));                        \
+        break;                                                                                                         \
+      }                                                                                                                \
+      if (relax == 2) {                                                                                                \
+        if (stage)                                                                                                     \
+          LAUNCH_TIMED(c, "pass_f_relax",                                                                              \
+                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, true, false, 2>), dim3(gr), dim3(kBlock),   \
+                                          lds, c->stream, PF_ARGS(Wr), c->rlist, cnt, c->rpart));                      \
+        else                                                                                                           \
+          LAUNCH_TIMED(c, "pass_f_relax",                                                                              \
+                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false, false, 2>), dim3(gr), dim3(kBlock),  \
+                                          0, c->stream, PF_ARGS(Wr), c->rlist, cnt, c->rpart));                        \
         break;                                                                                                         \
       }                                                                                                                \
     }                                                                                                                  \
@@ -1028,11 +1062,11 @@ int do_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want_w);
 // kernel also writes the primitive records of the new state (mixed precision).
 // relax (fused form only): the kernel also runs stage 1 of the surrogate
 // relaxation (fk_relax_mlp_begin before, fk_relax_mlp_finish after).
-int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want_w, bool relax = false) {
+int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want_w, int relax = 0) {
   if (relax) {
     const int nb = next_buffer(c);
-    int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 0>(c, irk, dt, dtp, c->U[nb], false, true)
-                                        : launch_pass_f_t<double, 0>(c, irk, dt, dtp, c->U[nb], false, true);
+    int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 0>(c, irk, dt, dtp, c->U[nb], false, relax)
+                                        : launch_pass_f_t<double, 0>(c, irk, dt, dtp, c->U[nb], false, relax);
     if (rc) return rc;
     c->ia = nb;
     c->w_valid = false;  // until stage 2 has written the listed cells' records
@@ -1124,9 +1158,9 @@ int ensure_rlist(fk_ctx *c) {
   c->rpart = nullptr;
   c->rlist_cap = 0;
   HIPCHK(hipMalloc(&c->rlist, (size_t)c->nc * sizeof(uint32_t)));
-  // [blocks][4] partials of the relaxation kernels, or [waves][2] of the
+  // [blocks][4] partials of the relaxation kernels, or [waves][2 or 3] of the
   // fused pass F (grid_xcd blocks of 4 waves)
-  HIPCHK(hipMalloc(&c->rpart, (size_t)grid_xcd(c->nc) * 8 * sizeof(unsigned long long)));
+  HIPCHK(hipMalloc(&c->rpart, (size_t)grid_xcd(c->nc) * 12 * sizeof(unsigned long long)));
   c->rlist_cap = c->nc;
   return FK_OK;
 }
@@ -1184,6 +1218,33 @@ int relax_fused_finish(fk_ctx *c, bool warm) {
                                                     c->stream, c->rpart, nw, cnt));
   int rc = fk_relax_mlp_finish(c, warm);
   if (rc) return rc;
+  hipLaunchKernelGGL(accum_counts, dim3(1), dim3(64), 0, c->stream, cnt);
+  HIPCHK(hipGetLastError());
+  ++c->relax_calls;
+  return FK_OK;
+}
+
+// The plain relaxation inside the last fused pass F: zeroed counts before it,
+// then its per-wave counts, the chemical solve of the listed cells and the
+// running totals (as do_relax).
+int relax_plain_begin(fk_ctx *c) {
+  int rc = ensure_rlist(c);
+  if (rc) return rc;
+  unsigned long long *cnt = (unsigned long long *)c->lcount;
+  LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_ull, dim3(1), dim3(64), 0, c->stream, cnt, 6));
+  return FK_OK;
+}
+
+int relax_plain_fused_finish(fk_ctx *c) {
+  unsigned long long *cnt = (unsigned long long *)c->lcount;
+  const int64_t nw = (int64_t)grid_xcd(c->nc) * (kBlock / 64);
+  LAUNCH_TIMED(c, "relax_count", hipLaunchKernelGGL(fk::reduce_partials<3>, dim3(fk::reduce_grid(nw)), dim3(256), 0,
+                                                    c->stream, c->rpart, nw, cnt));
+  float *W = (c->p.precision == FK_MIXED && c->prims_w) ? c->wprim : nullptr;
+  const unsigned gch = (unsigned)std::max<int64_t>(1, std::min<int64_t>(grid_for(c->nc), 1024));
+  LAUNCH_TIMED(c, "relax_chem", hipLaunchKernelGGL(relax_chem, dim3(gch), dim3(kBlock), 0, c->stream, c->p,
+                                                   c->U[c->ia], c->rlist, cnt, W));
+  c->w_valid = W != nullptr;
   hipLaunchKernelGGL(accum_counts, dim3(1), dim3(64), 0, c->stream, cnt);
   HIPCHK(hipGetLastError());
   ++c->relax_calls;
@@ -1548,9 +1609,12 @@ int fk_run_steps(fk_ctx *c, int nsteps, double *dt_last, double *phys_time) {
     // the last stage's state goes to the relaxation, which writes the
     // primitive records itself
     const bool relax_next = p.model == FK_MODEL_TWO_PHASE && p.relaxFlag == 1;
-    const bool mlp_relax = relax_next && (p.relaxMode == FK_RELAX_MLP || p.relaxMode == FK_RELAX_COLD);
     const bool warm = p.relaxMode == FK_RELAX_MLP;
-    bool relaxed_in_f = false;
+    // relaxation inside the last fused pass F: 1 surrogate stage 1, 2 plain
+    // (not while the plain relaxation emits training pairs)
+    const bool emitting = c->mlp.emit_stride > 0 && c->mlp.cap_data > c->mlp.n_data;
+    const int relax_kind = !relax_next ? 0 : (p.relaxMode == FK_RELAX_PLAIN ? (emitting ? 0 : 2) : 1);
+    int relaxed_in_f = 0;
     for (int irk = 0; irk < p.rkSteps; ++irk) {
       const bool want_w = irk < p.rkSteps - 1 || !relax_next;
       if ((rc = do_pass_g(c))) return rc;
@@ -1559,9 +1623,9 @@ int fk_run_steps(fk_ctx *c, int nsteps, double *dt_last, double *phys_time) {
         if ((rc = do_dt_device(c))) return rc;
         if ((rc = do_update(c, irk, 0.0, c->dscal, want_w))) return rc;
       } else {
-        // the last stage's fused pass F runs stage 1 of the surrogate relaxation
-        const bool fr = mlp_relax && c->fuse_relax && c->fuse_update && irk == p.rkSteps - 1;
-        if (fr && (rc = fk_relax_mlp_begin(c, warm))) return rc;
+        const int fr = (c->fuse_relax && c->fuse_update && irk == p.rkSteps - 1) ? relax_kind : 0;
+        if (fr == 1 && (rc = fk_relax_mlp_begin(c, warm))) return rc;
+        if (fr == 2 && (rc = relax_plain_begin(c))) return rc;
         if (p.adaptDt == 1) rc = do_pass_f_update(c, irk, 0.0, c->dscal, want_w, fr);
         else rc = do_pass_f_update(c, irk, p.dt, nullptr, want_w, fr);
         if (rc) return rc;
@@ -1572,8 +1636,10 @@ int fk_run_steps(fk_ctx *c, int nsteps, double *dt_last, double *phys_time) {
       LAUNCH_TIMED(c, "advance_time", hipLaunchKernelGGL(advance_time, dim3(1), dim3(64), 0, c->stream, c->dscal));
     else
       c->dt = p.dt;
-    if (relaxed_in_f) {
+    if (relaxed_in_f == 1) {
       if ((rc = relax_fused_finish(c, warm))) return rc;
+    } else if (relaxed_in_f == 2) {
+      if ((rc = relax_plain_fused_finish(c))) return rc;
     } else if (relax_next) {
       if ((rc = do_relax(c, nullptr, p.relaxMode))) return rc;
     }

@@ -520,6 +520,27 @@ FK_HD inline int screen_cell(const fk_params &P, double *u, int *chem, int *iter
   return 1;
 }
 
+// Plain relaxation of one cell, in place on u (relaxation.cu:40-75 with
+// thermRlx.cu:189-235): with chemFlag, a cell whose liquid is above Tsat(p_L)
+// and alpha_L > chemLimit is chemical and left for the chemical solve
+// (returns 1); every other cell gets the reference thermal Newton solve.
+// *iters and *tsat_iters count the Newton and Tsat iterations.
+FK_HD inline int plain_cell(const fk_params &P, double *u, int *iters, int *tsat_iters) {
+  int chem = 0, ts = 0;
+  if (P.chemFlag == 1) {
+    const double avf = clampd(P, u[0]);
+    const double rho = u[1] / avf;
+    const double enr = u[6] / (avf * rho);
+    const double tL = eos_T(P, 0, rho, enr);
+    const double pL = eos_p(P, 0, rho, enr);
+    const double Ts = tsat_dev(P, pL, tL, P.satTempUrf, &ts);
+    chem = (tL > Ts && avf > P.chemLimit) ? 1 : 0;
+  }
+  *tsat_iters = ts;
+  *iters = chem ? 0 : therm_dev(P, u);
+  return chem;
+}
+
 // Chemical (p-T-g) equilibrium of chemRlx (chemRlx.cu:52-76, 195-215) by
 // Newton with a forward-difference Jacobian (step 1e-7 |x|) and full steps,
 // from the warm start (alpha_k, rho_k, e_k), to |dx_i| <= tol |x_i|.

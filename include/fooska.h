@@ -186,10 +186,12 @@ int fk_set_debug(fk_ctx *ctx, int on);
  *     new state (fused pass F, relaxation) also write its FP32 primitive
  *     variables, which the next pass G gathers instead of converting the FP64
  *     conserved state of the cell and its neighbours.
- *  "fuse_relax" (default 1): with FK_RELAX_MLP or FK_RELAX_COLD in
- *     fk_run_steps, the last RK stage's fused pass F also runs stage 1 of the
- *     surrogate relaxation (classification, cold-start Newton) on the state
- *     it forms, instead of a separate kernel reading it back.
+ *  "fuse_relax" (default 1): in fk_run_steps the last RK stage's fused pass
+ *     F also runs the per-cell relaxation pass on the state it forms, instead
+ *     of a separate kernel reading it back: stage 1 of the surrogate
+ *     relaxation (FK_RELAX_MLP, FK_RELAX_COLD) or the plain Tsat
+ *     classification and thermal solve (FK_RELAX_PLAIN, except while it emits
+ *     training pairs); the chemical / listed cells follow in their own kernel.
  * Unknown names return FK_EINVAL. */
 int fk_set_tuning(fk_ctx *ctx, const char *name, int64_t value);
 

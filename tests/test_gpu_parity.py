@@ -374,11 +374,13 @@ def test_finite_rate_refuses_the_surrogate_relaxation(gpu):
 
 
 @pytest.mark.parametrize("precision", [_lib.FP64, _lib.MIXED])
-@pytest.mark.parametrize("mode", [_lib.RELAX_COLD, _lib.RELAX_MLP])
+@pytest.mark.parametrize("mode", [_lib.RELAX_COLD, _lib.RELAX_MLP, _lib.RELAX_PLAIN])
 def test_fused_relaxation_stage_equals_separate_kernel(gpu, precision, mode):
-    """fuse_relax: stage 1 of the surrogate relaxation inside the last RK
-    stage's fused pass F, or in its own kernel after it: bitwise equal states
-    and equal counts (cavitation: thermal and chemical cells; shock-bubble)."""
+    """fuse_relax: the per-cell relaxation pass (stage 1 of the surrogate
+    relaxation, or the plain thermal solve with its Tsat classification)
+    inside the last RK stage's fused pass F, or in its own kernel after it:
+    bitwise equal states and equal counts (cavitation: thermal and chemical
+    cells; shock-bubble)."""
     from fooskagpu_amd import surrogate
     rng = np.random.default_rng(0)
     x = (rng.normal(size=(1000, 3)) * [2.0, 3e5, 0.3] + [5.0, 1.4e6, 0.5]).astype(np.float32)
