@@ -340,6 +340,10 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
         if (i < nw) __builtin_amdgcn_global_load_lds((const void *)(src + i), (lds_ptr)(dst + k * kBlock + wbase), 4, 0, 0);
       }
     }
+    // the direct-to-LDS loads count as vector memory loads: wait for all of
+    // them explicitly before the barrier, so other waves' rows are in LDS
+    // whatever the compiler's lowering of the barrier
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
   }
   if (c >= nc) {
@@ -1167,6 +1171,19 @@ int ensure_rlist(fk_ctx *c) {
 
 namespace {
 
+// Checks and scratch of a relaxation of mode `mode`, done before a loop of
+// steps starts (the same checks run again inside each relaxation).
+int relax_preflight(fk_ctx *c, int mode) {
+  if (mode == FK_RELAX_MLP && !c->mlp.ready)
+    return fail(FK_ESTATE, "surrogate relaxation needs trained or loaded MLP weights");
+  if (mode == FK_RELAX_MLP || mode == FK_RELAX_COLD) {
+    const fk_params &P = c->p;
+    if (P.chemFlag == 1 && !((P.eta[0] - P.eta[1]) <= 0 && (P.Cp[1] - P.Cp[0]) >= 0))
+      return fail(FK_EINVAL, "surrogate classification needs eta_L <= eta_G and Cp_G >= Cp_L");
+  }
+  return ensure_rlist(c);
+}
+
 int do_relax(fk_ctx *c, fk_relax_stats *st, int mode = FK_RELAX_PLAIN) {
   if (c->p.model != FK_MODEL_TWO_PHASE) return fail(FK_EINVAL, "relaxation needs the two-phase model");
   unsigned long long *cnt = (unsigned long long *)c->lcount;
@@ -1597,9 +1614,10 @@ int fk_run_steps(fk_ctx *c, int nsteps, double *dt_last, double *phys_time) {
   if (nsteps < 0) return fail(FK_EINVAL, "negative step count");
   HIPCHK(hipSetDevice(c->device));
   const fk_params &p = c->p;
-  // a surrogate relaxation without weights fails here, before any stage ran
-  if (p.model == FK_MODEL_TWO_PHASE && p.relaxFlag == 1 && p.relaxMode == FK_RELAX_MLP && !c->mlp.ready && nsteps > 0)
-    return fail(FK_ESTATE, "surrogate relaxation needs trained or loaded MLP weights");
+  // every check and allocation of the relaxation happens here, before any
+  // stage runs, so a failure leaves the state and the clock untouched
+  if (p.model == FK_MODEL_TWO_PHASE && p.relaxFlag == 1 && nsteps > 0 && (rc = relax_preflight(c, p.relaxMode)))
+    return rc;
   // the device scalars hold dt and the physical time
   double h[2] = {c->dt, c->time};
   HIPCHK(hipMemcpy(c->dscal, h, sizeof h, hipMemcpyHostToDevice));

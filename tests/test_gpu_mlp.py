@@ -174,3 +174,16 @@ def test_surrogate_steps_without_weights_fail_before_any_stage(gpu):
         with pytest.raises(FkError, match="weights"):
             g.run_steps(2)
         assert np.array_equal(g.download_state(), c.U) and g.clock()[2] == 0
+
+
+def test_surrogate_classification_check_fails_before_any_stage(gpu):
+    """The exact classification of the surrogate relaxation needs
+    eta_L <= eta_G and Cp_G >= Cp_L: with Cp_G < Cp_L fk_run_steps refuses
+    before the first RK stage, leaving the state and the clock untouched."""
+    from fooskagpu_amd._lib import FkError
+    c = cases.bubble(n=8)
+    p = dict(c.params, relaxMode=_lib.RELAX_COLD, Cp=[c.params["Cp"][0], c.params["Cp"][0] * 0.5])
+    with GpuSolver(p, c.mesh, c.U) as g:
+        with pytest.raises(FkError, match="Cp_G"):
+            g.run_steps(2)
+        assert np.array_equal(g.download_state(), c.U) and g.clock()[2] == 0
