@@ -117,6 +117,7 @@ HIP_API = [
     ("fk_relaxation", ctypes.c_int, [_ctx, ctypes.c_int, ctypes.POINTER(FkRelaxStats)]),
     ("fk_relax_totals", ctypes.c_int, [_ctx, ctypes.POINTER(FkRelaxStats), ctypes.c_int]),
     ("fk_cons_calc", ctypes.c_int, [_ctx, c_double_p]),
+    ("fk_cons_calc_noalloc", ctypes.c_int, [_ctx, c_double_p]),
     ("fk_run_steps", ctypes.c_int, [_ctx, ctypes.c_int, c_double_p, c_double_p]),
     ("fk_sync", ctypes.c_int, [_ctx]),
     ("fk_get_clock", ctypes.c_int, [_ctx, c_double_p, c_double_p, ctypes.POINTER(ctypes.c_int64)]),

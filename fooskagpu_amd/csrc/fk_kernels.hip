@@ -43,6 +43,41 @@
 
 using namespace fkx;
 
+// Scratch of fk_cons_calc_noalloc: the per-block partial sums of the volume
+// integrals ([kConsBlocks][NEQ]) followed by the folded sums ([NEQ]).  Each
+// device holds its own copy for the life of the process (32.9 KB).  In host
+// code `scratch` only names the symbol: kernels and copies take the device
+// address from hipGetSymbolAddress, kept per device in g_scratch.
+constexpr int kConsBlocks = 256;
+constexpr int kScratchDoubles = kConsBlocks * 16 + 16;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+double *g_scratch[kMaxDevices] = {};
+std::mutex g_scratch_mu;
+
+// Device address of `scratch` on the current device `dev`, looked up once per
+// device (called by fk_create with that device current).
+int scratch_setup(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return fail(FK_EDEVICE, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch[dev]) return FK_OK;
+  void *ptr = nullptr;
+  hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    return fail(FK_EDEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch[dev] = static_cast<double *>(ptr);
+  return FK_OK;
+}
+
+double *scratch_of(int dev) {
+  if (dev < 0 || dev >= kMaxDevices) return nullptr;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  return g_scratch[dev];
+}
+}  // namespace
+
 namespace {
 
 // ------------------------------------------------------------------------
@@ -845,6 +880,18 @@ __global__ void __launch_bounds__(kBlock) integrate(int64_t nc, const double *__
   }
 }
 
+// Folds the per-block partials of integrate in block order, one thread per
+// variable: out[v] = ((p[0][v] + p[1][v]) + ...), the order of the host sum
+// of fk_cons_calc, so both give the same bits.
+template <int NEQ>
+__global__ void fold_partials(int nb, const double *__restrict__ part, double *__restrict__ out) {
+  const int v = threadIdx.x;
+  if (v >= NEQ) return;
+  double s = 0;
+  for (int b = 0; b < nb; ++b) s += part[b * NEQ + v];
+  out[v] = s;
+}
+
 __global__ void __launch_bounds__(kBlock) stream_copy(int64_t n16, const uint4 *__restrict__ a, uint4 *__restrict__ b) {
   for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n16; i += (int64_t)gridDim.x * kBlock) b[i] = a[i];
 }
@@ -1311,6 +1358,8 @@ int fk_create(int device, const fk_params *p, fk_ctx **out) {
   HIPCHK(hipGetDeviceCount(&ndev));
   if (device < 0 || device >= ndev) return fail(FK_EDEVICE, "no such HIP device");
   HIPCHK(hipSetDevice(device));
+  rc = scratch_setup(device);  // the no-allocation entry points' scratch on this device
+  if (rc) return rc;
   fk_ctx *c = new (std::nothrow) fk_ctx;
   if (!c) return fail(FK_ENOMEM, "out of host memory");
   c->device = device;
@@ -1591,7 +1640,7 @@ int fk_cons_calc(fk_ctx *c, double *out) {
   if (rc) return rc;
   if (!out) return fail(FK_EINVAL, "null argument");
   HIPCHK(hipSetDevice(c->device));
-  const int nb = 256;
+  const int nb = kConsBlocks;
   if (c->neq == 5)
     hipLaunchKernelGGL((integrate<5>), dim3(nb), dim3(kBlock), 0, c->stream, c->nc, c->U[c->ia], c->vol, c->red);
   else
@@ -1605,6 +1654,32 @@ int fk_cons_calc(fk_ctx *c, double *out) {
     for (int b = 0; b < nb; ++b) s += h[(size_t)(b * c->neq + v)];
     out[v] = s;
   }
+  return FK_OK;
+}
+
+int fk_cons_calc_noalloc(fk_ctx *c, double *out) {
+  int rc = require(c);
+  if (rc) return rc;
+  if (!out) return fail(FK_EINVAL, "null argument");
+  if ((int64_t)kConsBlocks * c->neq + c->neq > kScratchDoubles)
+    return fail(FK_ERANGE, "fk_cons_calc_noalloc: " + std::to_string(c->neq) + " variables exceed the scratch of " +
+                               std::to_string(kScratchDoubles) + " doubles");
+  HIPCHK(hipSetDevice(c->device));
+  double *ptr = scratch_of(c->device);
+  if (!ptr) return fail(FK_ESTATE, "scratch not set up on this device");
+  // partials [kConsBlocks][NEQ] at ptr, folded sums [NEQ] after them; every
+  // element read is written first in this call, so nothing needs clearing
+  double *sums = ptr + (int64_t)kConsBlocks * c->neq;
+  if (c->neq == 5) {
+    hipLaunchKernelGGL((integrate<5>), dim3(kConsBlocks), dim3(kBlock), 0, c->stream, c->nc, c->U[c->ia], c->vol, ptr);
+    hipLaunchKernelGGL((fold_partials<5>), dim3(1), dim3(64), 0, c->stream, kConsBlocks, ptr, sums);
+  } else {
+    hipLaunchKernelGGL((integrate<10>), dim3(kConsBlocks), dim3(kBlock), 0, c->stream, c->nc, c->U[c->ia], c->vol, ptr);
+    hipLaunchKernelGGL((fold_partials<10>), dim3(1), dim3(64), 0, c->stream, kConsBlocks, ptr, sums);
+  }
+  HIPCHK(hipGetLastError());
+  HIPCHK(hipMemcpyAsync(out, sums, (size_t)c->neq * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
   return FK_OK;
 }
 

@@ -143,6 +143,15 @@ int fk_relaxation(fk_ctx *ctx, int mode, fk_relax_stats *stats);
 int fk_relax_totals(fk_ctx *ctx, fk_relax_stats *stats, int reset);
 /* consCalc (consCalc.cu:36): out[i] = sum_c U_i V_c (i = 0 integrates 1 - alpha). */
 int fk_cons_calc(fk_ctx *ctx, double *out);
+/* fk_cons_calc without any allocation: no hipMalloc, hipFree or host heap
+ * use in the call.  No counterpart in the reference; it is for callers that
+ * cannot allocate (code under HIP graph capture, tight serving loops).  Its
+ * partial sums live in a fixed device array of 4112 doubles (32.9 KB) that
+ * each device holds until the process ends, set up by fk_create; results are
+ * bitwise those of fk_cons_calc.  Every context on one device shares that
+ * array: concurrent calls on one device race and must be serialised by the
+ * caller. */
+int fk_cons_calc_noalloc(fk_ctx *ctx, double *out);
 
 /* Whole time steps in explicitLoop order (dt is taken before the stage-0
  * update of every adaptDtItr-th step), with the relaxation of relaxMode
