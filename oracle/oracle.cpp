@@ -270,7 +270,11 @@ void face_flux(const fk_params &p, const double *WL, const double *WRin, const d
     if (p.hllcEnergy == 1)
       vecS[8] = ((SK - un) / (SK - SS)) * (consF[8] + (SS - un) * (rho * SS + press / (SK - un)));
     else
+#ifndef ORC_PLANT_ENERGY_P_SIGN
       vecS[8] = ((SK - un) / (SK - SS)) * (consF[8] + (SS - un) * (SS + press / (rho * (SK - un))));
+#else
+      vecS[8] = ((SK - un) / (SK - SS)) * (consF[8] + (SS - un) * (SS - press / (rho * (SK - un))));
+#endif
     vecS[9] = s[8] * (SK - un) / (SK - SS);
     // computeFluxHllc (DP computeRHS.cu:24-36)
     out.hllc[1] = cK * (vecS[1] - consF[1]);
@@ -401,6 +405,9 @@ double tsat(const fk_params &p, double P, double temp, double urf, int *iters) {
   double Bs = (eta1 - eta2) / (Cp2 - Cv2);
   double Cs = (Cp2 - Cp1) / (Cp2 - Cv2);
   double Ds = (Cp1 - Cv1) / (Cp2 - Cv2);
+#ifdef ORC_PLANT_TSAT_NO_D
+  Ds = 0;
+#endif
   double dt = 1.e-3, fun = 1, T = temp;
   int itr = 0;
   const double lpp = fkmath::log10(P + Pinf), lp = fkmath::log10(P);  // loop invariants
@@ -617,6 +624,7 @@ struct Chem {
 // checkUpdate (chemRlx.cu:139-162); urf arrives as float as in the reference
 void check_update(const fk_params &p, double xOld, double *x, double dx, int i, float urf, int *abortFlag, double rho) {
   if (i == 0 || i == 1) {
+#ifndef ORC_PLANT_NO_ALPHA_BOUND
     if (x[i] < 0) {
       while (x[i] < p.chemLimit && urf > 0.0005) {
         urf = urf / 2;
@@ -631,6 +639,7 @@ void check_update(const fk_params &p, double xOld, double *x, double dx, int i, 
     }
     if (x[i] < 0) *abortFlag = 1;
     if (x[i] > 1) *abortFlag = 1;
+#endif
   } else if (i == 2 || i == 3) {
     double ar = x[i - 2] * x[i];
     if (x[i] < 0) {
@@ -639,6 +648,7 @@ void check_update(const fk_params &p, double xOld, double *x, double dx, int i, 
         x[i] = xOld - urf * dx;
       }
     }
+#ifndef ORC_PLANT_NO_DENSITY_BOUND
     if (ar > rho) {
       while (ar > rho && urf > 0.0005) {
         urf = urf / 2;
@@ -646,13 +656,31 @@ void check_update(const fk_params &p, double xOld, double *x, double dx, int i, 
         ar = x[i - 2] * x[i];
       }
     }
-    if (x[i] < 0) *abortFlag = 1;
     if (ar > rho) *abortFlag = 1;
+#endif
+    if (x[i] < 0) *abortFlag = 1;
   }
 }
 
+// Record of the chemical Newton iterations, for the oracle's own tests
+// (orc_chem_trace); per iteration kChemTraceCols doubles:
+//   [0, 6)   x before the update, [6, 9) its residuals,
+//   [9]      abortCheck: 0 x not eligible (abort flag or bounds), 1 eligible,
+//            2 eligible and taken as the best so far,
+//   [10]     the FP32 residual norm abortCheck compares,
+//   [11, 17) x after the update (with checkUpdate's halving),
+//   [17]     1 if the update raised the abort flag,
+//   [18, 20) the non-primary phase's volume fraction and density after the
+//            full Newton step, before checkUpdate.
+constexpr int kChemTraceCols = 20;
+struct ChemTrace {
+  double *rows;
+  int max_rows, n_rows;
+  int aborted;  // the solve ended on the abort path (abortX restored)
+};
+
 // chemRlx for one cell (chemRlx.cu:217-267 with helpers :4-215)
-int chem_cell(const fk_params &p, double *U) {
+int chem_cell(const fk_params &p, double *U, ChemTrace *tr = nullptr) {
   const int itrMax = p.chemItrMax;
   const double urf = p.chemUrf;
   double x[6], abortX[6];
@@ -712,11 +740,20 @@ int chem_cell(const fk_params &p, double *U) {
     inv[2][1] = -id * (jac[0][0] * jac[2][1] - jac[0][1] * jac[2][0]);
     inv[2][2] = id * (jac[0][0] * jac[1][1] - jac[0][1] * jac[1][0]);
     // abortCheck (chemRlx.cu:125-137)
+    double *row = (tr && tr->n_rows < tr->max_rows) ? tr->rows + (size_t)tr->n_rows++ * kChemTraceCols : nullptr;
+    if (row) {
+      for (int i = 0; i < 6; i++) row[i] = x[i];
+      for (int i = 0; i < 3; i++) row[6 + i] = res[i];
+      row[9] = 0;
+      row[10] = (float)std::sqrt(res[0] * res[0] + res[1] * res[1] + res[2] * res[2]);
+    }
     if (abortFlag == 0 && x[0] > 0.0 && x[0] < 1.0 && x[1] > 0.0 && x[1] < 1.0 && x[2] > 0.0 && x[3] > 0.0) {
       float check = (float)std::sqrt(res[0] * res[0] + res[1] * res[1] + res[2] * res[2]);
+      if (row) row[9] = 1;
       if (check < minRes) {
         minRes = check;
         for (int i = 0; i < 6; i++) abortX[i] = x[i];
+        if (row) row[9] = 2;
       }
     }
     // updateSolution (chemRlx.cu:164-193): abortFlag is a by-value copy here
@@ -727,15 +764,24 @@ int chem_cell(const fk_params &p, double *U) {
         double dx = 0;
         for (int j = 0; j < 3; j++) dx += inv[iv][j] * res[j];
         x[i] = x[i] - urf * dx;
+        if (row && (i == 0 || i == 1)) row[18] = x[i];
+        if (row && (i == 2 || i == 3)) row[19] = x[i];
         check_update(p, x[i] + urf * dx, x, dx, i, (float)urf, &abortLocal, ch.rhoMix);
         iv++;
       }
+    if (row) {
+      for (int i = 0; i < 6; i++) row[11 + i] = x[i];
+      row[17] = abortLocal;
+    }
     int check = 0;
     for (int j = 0; j < 3; j++)
       if (std::fabs(res[j]) > 1.e-1) check += 1;
     if (check == 0) conv = true;
     if (abortLocal == 1 || itr > itrMax) {
+#ifndef ORC_PLANT_ABORT_KEEPS_LAST
       for (int i = 0; i < 6; i++) x[i] = abortX[i];
+#endif
+      if (tr) tr->aborted = 1;
       conv = true;
     }
   }
@@ -1303,6 +1349,14 @@ void orc_fnt_press_rlx(const fk_params *p, const double *avf, const double *rho,
   fnt_press_rlx(*p, avf, rho, ener, dP, dT);
 }
 double orc_gibbs(const fk_params *p, int k, double P, double T) { return gibbs(*p, k, P, T); }
+
+int orc_chem_trace(const fk_params *p, double *U, double *rows, int max_rows, int *n_rows, int *aborted) {
+  ChemTrace tr{rows, max_rows, 0, 0};
+  int it = chem_cell(*p, U, &tr);
+  if (n_rows) *n_rows = tr.n_rows;
+  if (aborted) *aborted = tr.aborted;
+  return it;
+}
 
 int orc_relax_cell(const fk_params *p, double *U, int *iters, int *tsat_iters) {
   int it = 0, ts = 0;

@@ -85,6 +85,11 @@ double orc_gibbs(const fk_params *p, int k, double P, double T);
 /* Relax one cell's conserved state in place; returns 0 thermal, 1 chemical;
  * iters = Newton iterations. */
 int orc_relax_cell(const fk_params *p, double *U, int *iters, int *tsat_iters);
+/* The chemical relaxation (chemRlx) of one cell with its Newton iterations
+ * recorded: rows[max_rows][20] as kChemTraceCols in oracle.cpp, *n_rows the
+ * iterations recorded, *aborted 1 when the solve ended on the abort path.
+ * Returns the iteration count.  For the oracle's own tests. */
+int orc_chem_trace(const fk_params *p, double *U, double *rows, int max_rows, int *n_rows, int *aborted);
 
 #ifdef __cplusplus
 }

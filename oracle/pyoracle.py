@@ -19,13 +19,39 @@ def _params_type():
     return FkParams
 
 
+_override = []  # a library loaded by `variant`, used in place of the built one
+
+
 def lib(omp=False):
+    if _override:
+        return _override[-1]
     name = "liboracle_omp.so" if omp else "liboracle.so"
     if name in _libs:
         return _libs[name]
     path = os.path.join(HERE, name)
     if not os.path.exists(path):
         raise RuntimeError(f"{path} not built; run make -C oracle")
+    _libs[name] = _load(path)
+    return _libs[name]
+
+
+class variant:
+    """Context in which every oracle call (Oracle objects created inside it
+    and the pointwise helpers) uses the library at `path`: a build of
+    oracle.cpp with a planted mistake, for tests that show a pin catches it."""
+
+    def __init__(self, path):
+        self.L = _load(path)
+
+    def __enter__(self):
+        _override.append(self.L)
+        return self
+
+    def __exit__(self, *exc):
+        _override.pop()
+
+
+def _load(path):
     L = ctypes.CDLL(path)
     P = ctypes.POINTER(_params_type())
     vp = ctypes.c_void_p
@@ -60,11 +86,12 @@ def lib(omp=False):
         "orc_fnt_press_rlx": (None, [P, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_float),
                                      ctypes.POINTER(ctypes.c_float)]),
         "orc_gibbs": (ctypes.c_double, [P, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+        "orc_chem_trace": (ctypes.c_int, [P, _dp, _dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int)]),
     }
     for n, (r, a) in sig.items():
         f = getattr(L, n)
         f.restype, f.argtypes = r, a
-    _libs[name] = L
     return L
 
 
@@ -248,3 +275,20 @@ def relax_cell(params, U):
     it, ts = ctypes.c_int(), ctypes.c_int()
     kind = lib().orc_relax_cell(ctypes.byref(_P(params)), _d(U), ctypes.byref(it), ctypes.byref(ts))
     return U, kind, it.value, ts.value
+
+
+CHEM_TRACE_COLS = 20
+
+
+def chem_trace(params, U, max_rows=256):
+    """Chemical Newton solve of one cell with its iterations recorded:
+    returns (relaxed U, iterations, rows[n][18], aborted); row layout as
+    kChemTraceCols in oracle.cpp (x before the update, residuals, abortCheck
+    flag 0 not eligible / 1 eligible / 2 new best, FP32 residual norm, x after
+    the update, abort flag, the non-primary phase's volume fraction and density
+    after the full step, before checkUpdate)."""
+    U = np.array(U, dtype=np.float64)
+    rows = np.zeros((max_rows, CHEM_TRACE_COLS))
+    n, ab = ctypes.c_int(), ctypes.c_int()
+    it = lib().orc_chem_trace(ctypes.byref(_P(params)), _d(U), _d(rows), max_rows, ctypes.byref(n), ctypes.byref(ab))
+    return U, it, rows[:n.value].copy(), bool(ab.value)
