@@ -1,19 +1,25 @@
-// Host mesh and case library: tetrahedral box generator, Gambit neutral file
-// I/O, face numbering and geometry in the reference solver's conventions, and
-// initial conditions.  C ABI in include/fooska_mesh.h.
+// Host mesh and case library: box generator (tetrahedra, or hexahedra,
+// prisms, pyramids and tetrahedra mixed), Gambit neutral file I/O, face
+// numbering and geometry in the reference solver's conventions, and initial
+// conditions.  C ABI in include/fooska_mesh.h.
 //
 // Reference conventions followed here (one rank, so no METIS/halo maps):
-//  * tet face -> element-node table (meshProperties.cu:90-105, the nLclNodes==4
-//    rows): face 0 = (1,0,2), face 1 = (0,1,3), face 2 = (1,2,3), face 3 = (2,0,3)
-//  * face centres summed in that node order and divided by 3, cell centre =
-//    (n0+n1+n2+n3)/4 (translator.cu:283-305)
-//  * unit normal from the first three face nodes (meshProperties.cu:51-63),
-//    tangent t1 along the first edge and t2 = n x t1 (:65-82), area of the
-//    triangle (:84-100), signed tet volume (:102-137)
-//  * faces are numbered by walking cells in order and local faces 0..3,
+//  * element types by node count, 4 tet, 5 pyramid, 6 prism, 8 hex, with the
+//    face -> element-node tables of meshProperties.cu:4-49 (fcNd2ElNd,
+//    h_nFaceNodes); element nodes are stored 8 per cell as elNd2GlbNd
+//  * face centres summed in translator.cu's node order per element type and
+//    divided by the face's node count, cell centre = mean of the element's
+//    nodes (translator.cu:122-293)
+//  * unit normal from the first three fcNd2ElNd nodes of the face
+//    (meshProperties.cu:51-63), tangent t1 along the first edge and
+//    t2 = n x t1 (:65-82), area as the fan of triangles from the first node
+//    (:84-100), volume as the sum of the sub-tets of the element that share
+//    its node 0 (:102-137, tables :168-171)
+//  * faces are numbered by walking cells in order and their local faces,
 //    creating a face when the neighbour is a boundary or not yet visited, so
 //    the owner is always the lower cell index (buildLocalMesh.cu:244-343)
-//  * BC tags 45/26/13/42/51 -> codes 1..5 (translator.cu:299-349)
+//  * BC tags 45/26/13/42/51 -> codes 1..5, element face numbers of the
+//    boundary sets = local face + 1 (translator.cu:295-349)
 // Neighbours are matched by exact sorted node ids instead of the reference's
 // face-centre tolerance search (meshConnectivity.cu:97-128).
 
@@ -45,8 +51,39 @@ inline uint64_t splitmix64(uint64_t &s) {
 }
 inline double uniform01(uint64_t &s) { return (double)(splitmix64(s) >> 11) * 0x1.0p-53; }
 
-// Tet face -> element node table (reference meshProperties.cu:100-105).
-const int kFaceNode[4][3] = {{1, 0, 2}, {0, 1, 3}, {1, 2, 3}, {2, 0, 3}};
+constexpr int kNd = 8;  // node slots per cell (elNd2GlbNd stride)
+constexpr int kFc = 6;  // face slots per cell
+
+// One element type of the reference.
+struct ElemType {
+  int nnode, nface;
+  int fnn[kFc];      // nodes per face (h_nFaceNodes, meshProperties.cu:3-12)
+  int fn[kFc][4];    // fcNd2ElNd (meshProperties.cu:15-49): normal, area, tangent order
+  int fc[kFc][4];    // node order of the face-centre sums (translator.cu:122-293)
+  int ntet;          // sub-tets of the volume, all with node 0 (meshProperties.cu:168-171)
+  int vn[6][3];
+};
+const ElemType kTet = {4, 4, {3, 3, 3, 3, 0, 0},
+                       {{1, 0, 2}, {0, 1, 3}, {1, 2, 3}, {2, 0, 3}},
+                       {{1, 0, 2}, {0, 1, 3}, {1, 2, 3}, {2, 0, 3}},
+                       1, {{1, 2, 3}}};
+const ElemType kPyramid = {5, 5, {4, 3, 3, 3, 3, 0},
+                           {{0, 2, 3, 1}, {0, 1, 4}, {1, 3, 4}, {3, 2, 4}, {2, 0, 4}},
+                           {{0, 2, 3, 1}, {0, 1, 4}, {1, 3, 4}, {3, 2, 4}, {2, 0, 4}},
+                           2, {{1, 3, 4}, {3, 2, 4}}};
+const ElemType kPrism = {6, 5, {4, 4, 4, 3, 3, 0},
+                         {{0, 1, 4, 3}, {2, 5, 4, 1}, {0, 3, 5, 2}, {0, 2, 1}, {3, 4, 5}},
+                         {{0, 1, 4, 3}, {1, 2, 5, 4}, {2, 0, 3, 5}, {0, 2, 1}, {3, 4, 5}},
+                         3, {{1, 2, 5}, {1, 5, 4}, {4, 5, 3}}};
+const ElemType kHex = {8, 6, {4, 4, 4, 4, 4, 4},
+                       {{0, 1, 5, 4}, {3, 7, 5, 1}, {2, 6, 7, 3}, {2, 0, 4, 6}, {2, 3, 1, 0}, {6, 4, 5, 7}},
+                       {{0, 1, 5, 4}, {1, 3, 7, 5}, {3, 2, 6, 7}, {2, 0, 4, 6}, {1, 0, 2, 3}, {4, 5, 7, 6}},
+                       6, {{1, 3, 5}, {5, 3, 7}, {5, 7, 4}, {3, 2, 7}, {7, 2, 6}, {7, 6, 4}}};
+
+inline const ElemType &etype(int nnode) {
+  return nnode == 4 ? kTet : nnode == 5 ? kPyramid : nnode == 6 ? kPrism : kHex;
+}
+inline bool valid_nnode(long n) { return n == 4 || n == 5 || n == 6 || n == 8; }
 
 }  // namespace
 
@@ -54,11 +91,12 @@ struct fkm_mesh {
   int64_t nc = 0, nf = 0, nn = 0, nbf = 0;
   std::vector<double> nodes;     // [nn][3]
   std::vector<int64_t> topo;     // node -> topological id (periodic merge); empty = identity
-  std::vector<int32_t> el2nd;    // [nc][4]
-  std::vector<int32_t> elbc;     // [nc][4] boundary code per local face (0 = none)
-  std::vector<int32_t> nb;       // [nc][4] neighbour cell per local face, -1 boundary
-  std::vector<int8_t> nbface;    // [nc][4] local face index in the neighbour
-  std::vector<double> elshift;   // [nc][4][3] periodic shift per local face (empty if none)
+  std::vector<int32_t> el2nd;    // [nc][8] element nodes (elNd2GlbNd), -1 past the element's nodes
+  std::vector<int32_t> nnode;    // [nc] 4 tet, 5 pyramid, 6 prism, 8 hex
+  std::vector<int32_t> elbc;     // [nc][6] boundary code per local face (0 = none)
+  std::vector<int32_t> nb;       // [nc][6] neighbour cell per local face, -1 boundary or no face
+  std::vector<int8_t> nbface;    // [nc][6] local face index in the neighbour
+  std::vector<double> elshift;   // [nc][6][3] periodic shift per local face (empty if none)
   // reference MESH arrays
   std::vector<int32_t> fc2el, fc2elfc, bc;
   std::vector<double> n, nt1, nt2, area, fc_center, shift, cell_center, volume;
@@ -71,58 +109,62 @@ int64_t topo_of(const fkm_mesh &m, int64_t node) {
   return m.topo.empty() ? node : m.topo[(size_t)node];
 }
 
-// Match tet faces by their sorted topological node ids.  Faces are bucketed
-// by the smallest id (counting sort), so the whole match is O(faces).
+inline int nface_of(const fkm_mesh &m, int64_t c) { return etype(m.nnode[(size_t)c]).nface; }
+
+// Match faces by their sorted topological node ids (3 for triangles, 4 for
+// quadrilaterals; a triangle never matches a quadrilateral).  Faces are
+// bucketed by the smallest id (counting sort), so the whole match is
+// O(faces).
 int match_neighbours(fkm_mesh &m) {
   const int64_t nc = m.nc;
   int64_t ntopo = 0;
   for (int64_t i = 0; i < m.nn; ++i) ntopo = std::max(ntopo, topo_of(m, i) + 1);
   struct Key {
-    int64_t b, c;
+    int64_t b, c, d;
     int64_t cell;
     int8_t face;
   };
   std::vector<int64_t> start((size_t)ntopo + 1, 0);
-  auto face_key = [&](int64_t c, int j, int64_t &a, int64_t &b, int64_t &cc) {
-    int64_t v[3];
-    for (int k = 0; k < 3; ++k) v[k] = topo_of(m, m.el2nd[(size_t)(c * 4 + kFaceNode[j][k])]);
-    std::sort(v, v + 3);
-    a = v[0];
-    b = v[1];
-    cc = v[2];
+  auto face_key = [&](int64_t c, int j, int64_t v[4]) {
+    const ElemType &t = etype(m.nnode[(size_t)c]);
+    const int k = t.fnn[j];
+    for (int q = 0; q < k; ++q) v[q] = topo_of(m, m.el2nd[(size_t)(c * kNd + t.fn[j][q])]);
+    std::sort(v, v + k);
+    if (k == 3) v[3] = -1;
   };
   for (int64_t c = 0; c < nc; ++c)
-    for (int j = 0; j < 4; ++j) {
-      int64_t a, b, cc;
-      face_key(c, j, a, b, cc);
-      start[(size_t)a + 1]++;
+    for (int j = 0; j < nface_of(m, c); ++j) {
+      int64_t v[4];
+      face_key(c, j, v);
+      start[(size_t)v[0] + 1]++;
     }
   for (int64_t i = 0; i < ntopo; ++i) start[(size_t)i + 1] += start[(size_t)i];
-  std::vector<Key> keys((size_t)(nc * 4));
+  std::vector<Key> keys((size_t)start[(size_t)ntopo]);
   std::vector<int64_t> fill(start.begin(), start.end() - 1);
   for (int64_t c = 0; c < nc; ++c)
-    for (int j = 0; j < 4; ++j) {
-      int64_t a, b, cc;
-      face_key(c, j, a, b, cc);
-      keys[(size_t)fill[(size_t)a]++] = Key{b, cc, c, (int8_t)j};
+    for (int j = 0; j < nface_of(m, c); ++j) {
+      int64_t v[4];
+      face_key(c, j, v);
+      keys[(size_t)fill[(size_t)v[0]]++] = Key{v[1], v[2], v[3], c, (int8_t)j};
     }
-  m.nb.assign((size_t)(nc * 4), -1);
-  m.nbface.assign((size_t)(nc * 4), -1);
+  m.nb.assign((size_t)(nc * kFc), -1);
+  m.nbface.assign((size_t)(nc * kFc), -1);
   for (int64_t a = 0; a < ntopo; ++a) {
     Key *lo = keys.data() + start[(size_t)a], *hi = keys.data() + start[(size_t)a + 1];
     std::sort(lo, hi, [](const Key &x, const Key &y) {
       if (x.b != y.b) return x.b < y.b;
       if (x.c != y.c) return x.c < y.c;
+      if (x.d != y.d) return x.d < y.d;
       return x.cell < y.cell;
     });
     for (Key *p = lo; p < hi;) {
       Key *q = p + 1;
-      while (q < hi && q->b == p->b && q->c == p->c) ++q;
+      while (q < hi && q->b == p->b && q->c == p->c && q->d == p->d) ++q;
       if (q - p == 2) {
-        m.nb[(size_t)(p->cell * 4 + p->face)] = (int32_t)p[1].cell;
-        m.nbface[(size_t)(p->cell * 4 + p->face)] = p[1].face;
-        m.nb[(size_t)(p[1].cell * 4 + p[1].face)] = (int32_t)p->cell;
-        m.nbface[(size_t)(p[1].cell * 4 + p[1].face)] = p->face;
+        m.nb[(size_t)(p->cell * kFc + p->face)] = (int32_t)p[1].cell;
+        m.nbface[(size_t)(p->cell * kFc + p->face)] = p[1].face;
+        m.nb[(size_t)(p[1].cell * kFc + p[1].face)] = (int32_t)p->cell;
+        m.nbface[(size_t)(p[1].cell * kFc + p[1].face)] = p->face;
       } else if (q - p > 2) {
         return fail(FK_EINVAL, "non-manifold mesh: a face is shared by more than two cells");
       }
@@ -139,7 +181,7 @@ std::vector<int64_t> rcm_order(const fkm_mesh &m) {
   const int64_t nc = m.nc > 0 ? m.nc : 0;
   std::vector<int8_t> deg((size_t)nc, 0);
   for (int64_t c = 0; c < nc; ++c)
-    for (int j = 0; j < 4; ++j) deg[(size_t)c] += (m.nb[(size_t)(c * 4 + j)] >= 0);
+    for (int j = 0; j < kFc; ++j) deg[(size_t)c] += (m.nb[(size_t)(c * kFc + j)] >= 0);
   std::vector<int64_t> level((size_t)nc, -1);
   std::vector<int64_t> queue;
   queue.reserve((size_t)nc);
@@ -150,8 +192,8 @@ std::vector<int64_t> rcm_order(const fkm_mesh &m) {
     level[(size_t)s] = 0;
     for (size_t h = 0; h < queue.size(); ++h) {
       int64_t c = queue[h];
-      for (int j = 0; j < 4; ++j) {
-        int64_t d = m.nb[(size_t)(c * 4 + j)];
+      for (int j = 0; j < kFc; ++j) {
+        int64_t d = m.nb[(size_t)(c * kFc + j)];
         if (d >= 0 && level[(size_t)d] < 0) {
           level[(size_t)d] = level[(size_t)c] + 1;
           queue.push_back(d);
@@ -186,10 +228,10 @@ std::vector<int64_t> rcm_order(const fkm_mesh &m) {
     done[(size_t)s] = 1;
     for (size_t h = head; h < perm.size(); ++h) {
       int64_t c = perm[h];
-      int64_t nbs[4];
+      int64_t nbs[kFc];
       int k = 0;
-      for (int j = 0; j < 4; ++j) {
-        int64_t d = m.nb[(size_t)(c * 4 + j)];
+      for (int j = 0; j < kFc; ++j) {
+        int64_t d = m.nb[(size_t)(c * kFc + j)];
         if (d >= 0 && !done[(size_t)d]) nbs[k++] = d;
       }
       std::sort(nbs, nbs + k, [&](int64_t x, int64_t y) {
@@ -229,9 +271,10 @@ std::vector<int64_t> morton_order(const fkm_mesh &m) {
   for (int64_t c = 0; c < nc; ++c) {
     uint64_t code = 0;
     for (int a = 0; a < 3; ++a) {
+      const int nn = m.nnode[(size_t)c];
       double x = 0;
-      for (int q = 0; q < 4; ++q) x += m.nodes[(size_t)(m.el2nd[(size_t)(c * 4 + q)] * 3 + a)];
-      x *= 0.25;
+      for (int q = 0; q < nn; ++q) x += m.nodes[(size_t)(m.el2nd[(size_t)(c * kNd + q)] * 3 + a)];
+      x *= 1.0 / nn;
       const double span = hi[a] > lo[a] ? hi[a] - lo[a] : 1.0;
       double f = (x - lo[a]) / span;
       f = std::min(std::max(f, 0.0), 1.0);
@@ -249,23 +292,26 @@ void apply_order(fkm_mesh &m, const std::vector<int64_t> &perm) {
   const int64_t nc = m.nc;
   std::vector<int64_t> inv((size_t)nc);
   for (int64_t i = 0; i < nc; ++i) inv[(size_t)perm[(size_t)i]] = i;
-  std::vector<int32_t> el2nd((size_t)(nc * 4)), elbc((size_t)(nc * 4)), nb((size_t)(nc * 4));
-  std::vector<int8_t> nbface((size_t)(nc * 4));
+  std::vector<int32_t> el2nd((size_t)(nc * kNd)), nnode((size_t)nc), elbc((size_t)(nc * kFc)), nb((size_t)(nc * kFc));
+  std::vector<int8_t> nbface((size_t)(nc * kFc));
   std::vector<double> elshift;
-  if (!m.elshift.empty()) elshift.resize((size_t)(nc * 12));
+  if (!m.elshift.empty()) elshift.resize((size_t)(nc * kFc * 3));
   for (int64_t i = 0; i < nc; ++i) {
     int64_t o = perm[(size_t)i];
-    for (int j = 0; j < 4; ++j) {
-      el2nd[(size_t)(i * 4 + j)] = m.el2nd[(size_t)(o * 4 + j)];
-      elbc[(size_t)(i * 4 + j)] = m.elbc[(size_t)(o * 4 + j)];
-      int32_t d = m.nb[(size_t)(o * 4 + j)];
-      nb[(size_t)(i * 4 + j)] = d >= 0 ? (int32_t)inv[(size_t)d] : -1;
-      nbface[(size_t)(i * 4 + j)] = m.nbface[(size_t)(o * 4 + j)];
+    nnode[(size_t)i] = m.nnode[(size_t)o];
+    for (int q = 0; q < kNd; ++q) el2nd[(size_t)(i * kNd + q)] = m.el2nd[(size_t)(o * kNd + q)];
+    for (int j = 0; j < kFc; ++j) {
+      elbc[(size_t)(i * kFc + j)] = m.elbc[(size_t)(o * kFc + j)];
+      int32_t d = m.nb[(size_t)(o * kFc + j)];
+      nb[(size_t)(i * kFc + j)] = d >= 0 ? (int32_t)inv[(size_t)d] : -1;
+      nbface[(size_t)(i * kFc + j)] = m.nbface[(size_t)(o * kFc + j)];
       if (!elshift.empty())
-        for (int k = 0; k < 3; ++k) elshift[(size_t)(i * 12 + j * 3 + k)] = m.elshift[(size_t)(o * 12 + j * 3 + k)];
+        for (int k = 0; k < 3; ++k)
+          elshift[(size_t)((i * kFc + j) * 3 + k)] = m.elshift[(size_t)((o * kFc + j) * 3 + k)];
     }
   }
   m.el2nd.swap(el2nd);
+  m.nnode.swap(nnode);
   m.elbc.swap(elbc);
   m.nb.swap(nb);
   m.nbface.swap(nbface);
@@ -278,8 +324,8 @@ int build_faces_and_geometry(fkm_mesh &m) {
   const int64_t nc = m.nc;
   int64_t nf = 0, nbf = 0;
   for (int64_t c = 0; c < nc; ++c)
-    for (int j = 0; j < 4; ++j) {
-      int32_t d = m.nb[(size_t)(c * 4 + j)];
+    for (int j = 0; j < nface_of(m, c); ++j) {
+      int32_t d = m.nb[(size_t)(c * kFc + j)];
       if (d < 0) {
         ++nf;
         ++nbf;
@@ -308,22 +354,25 @@ int build_faces_and_geometry(fkm_mesh &m) {
   const double *X = m.nodes.data();
   int64_t f = 0;
   for (int64_t c = 0; c < nc; ++c) {
-    for (int j = 0; j < 4; ++j) {
-      int32_t d = m.nb[(size_t)(c * 4 + j)];
+    const ElemType &t = etype(m.nnode[(size_t)c]);
+    const int32_t *e = &m.el2nd[(size_t)(c * kNd)];
+    for (int j = 0; j < t.nface; ++j) {
+      int32_t d = m.nb[(size_t)(c * kFc + j)];
       if (!(d < 0 || d > c)) continue;
       m.fc2el[(size_t)(f * 2 + 0)] = (int32_t)c;
       m.fc2el[(size_t)(f * 2 + 1)] = d;
       m.fc2elfc[(size_t)(f * 2 + 0)] = j;
-      m.fc2elfc[(size_t)(f * 2 + 1)] = d >= 0 ? m.nbface[(size_t)(c * 4 + j)] : -1;
-      m.bc[(size_t)f] = d >= 0 ? 0 : m.elbc[(size_t)(c * 4 + j)];
-      double x[3], y[3], z[3];
-      for (int k = 0; k < 3; ++k) {
-        int64_t nd = m.el2nd[(size_t)(c * 4 + kFaceNode[j][k])];
-        x[k] = X[nd * 3 + 0];
-        y[k] = X[nd * 3 + 1];
-        z[k] = X[nd * 3 + 2];
+      m.fc2elfc[(size_t)(f * 2 + 1)] = d >= 0 ? m.nbface[(size_t)(c * kFc + j)] : -1;
+      m.bc[(size_t)f] = d >= 0 ? 0 : m.elbc[(size_t)(c * kFc + j)];
+      const int k = t.fnn[j];
+      double x[4], y[4], z[4];
+      for (int q = 0; q < k; ++q) {
+        int64_t nd = e[t.fn[j][q]];
+        x[q] = X[nd * 3 + 0];
+        y[q] = X[nd * 3 + 1];
+        z[q] = X[nd * 3 + 2];
       }
-      // normalVector (meshProperties.cu:51-63)
+      // normalVector (meshProperties.cu:51-63), first three face nodes
       double c0 = (y[1] - y[0]) * (z[2] - z[0]) - (y[2] - y[0]) * (z[1] - z[0]);
       double c1 = (x[1] - x[0]) * (z[2] - z[0]) - (x[2] - x[0]) * (z[1] - z[0]);
       double c2 = (x[1] - x[0]) * (y[2] - y[0]) - (x[2] - x[0]) * (y[1] - y[0]);
@@ -331,39 +380,48 @@ int build_faces_and_geometry(fkm_mesh &m) {
       m.n[(size_t)(f * 3 + 0)] = c0 / norm;
       m.n[(size_t)(f * 3 + 1)] = -c1 / norm;
       m.n[(size_t)(f * 3 + 2)] = c2 / norm;
-      // computeArea, one triangle (meshProperties.cu:84-100)
-      {
-        double dx1 = x[1] - x[0], dx2 = x[2] - x[0];
-        double dy1 = y[1] - y[0], dy2 = y[2] - y[0];
-        double dz1 = z[1] - z[0], dz2 = z[2] - z[0];
+      // computeArea (meshProperties.cu:84-100): triangles (0, 1+i, 2+i)
+      double ar = 0.0;
+      for (int it = 0; it < k - 2; ++it) {
+        double dx1 = x[1 + it] - x[0], dx2 = x[2 + it] - x[0];
+        double dy1 = y[1 + it] - y[0], dy2 = y[2 + it] - y[0];
+        double dz1 = z[1 + it] - z[0], dz2 = z[2 + it] - z[0];
         double a0 = dy1 * dz2 - dz1 * dy2, a1 = dz1 * dx2 - dx1 * dz2, a2 = dx1 * dy2 - dy1 * dx2;
-        m.area[(size_t)f] = std::fabs(0.5 * std::sqrt(a0 * a0 + a1 * a1 + a2 * a2));
+        ar += std::fabs(0.5 * std::sqrt(a0 * a0 + a1 * a1 + a2 * a2));
       }
+      m.area[(size_t)f] = ar;
       // face centre in translator order
-      m.fc_center[(size_t)(f * 3 + 0)] = (x[0] + x[1] + x[2]) / 3;
-      m.fc_center[(size_t)(f * 3 + 1)] = (y[0] + y[1] + y[2]) / 3;
-      m.fc_center[(size_t)(f * 3 + 2)] = (z[0] + z[1] + z[2]) / 3;
+      for (int a = 0; a < 3; ++a) {
+        double sum = X[(int64_t)e[t.fc[j][0]] * 3 + a];
+        for (int q = 1; q < k; ++q) sum = sum + X[(int64_t)e[t.fc[j][q]] * 3 + a];
+        m.fc_center[(size_t)(f * 3 + a)] = sum / k;
+      }
       if (!m.elshift.empty() && d >= 0)
-        for (int k = 0; k < 3; ++k) m.shift[(size_t)(f * 3 + k)] = m.elshift[(size_t)(c * 12 + j * 3 + k)];
+        for (int a = 0; a < 3; ++a) m.shift[(size_t)(f * 3 + a)] = m.elshift[(size_t)((c * kFc + j) * 3 + a)];
       ++f;
     }
-    // cell centre (translator.cu:283-285) and volume (meshProperties.cu:102-137)
-    const int32_t *e = &m.el2nd[(size_t)(c * 4)];
-    for (int k = 0; k < 3; ++k)
-      m.cell_center[(size_t)(c * 3 + k)] = (X[e[0] * 3 + k] + X[e[1] * 3 + k] + X[e[2] * 3 + k] + X[e[3] * 3 + k]) / 4;
-    double dx[3], dy[3], dz[3];
-    for (int in = 0; in < 3; ++in) {
-      dx[in] = X[e[in + 1] * 3 + 0] - X[e[0] * 3 + 0];
-      dy[in] = X[e[in + 1] * 3 + 1] - X[e[0] * 3 + 1];
-      dz[in] = X[e[in + 1] * 3 + 2] - X[e[0] * 3 + 2];
+    // cell centre (translator.cu, mean of the element nodes in order) and
+    // volume (meshProperties.cu:102-137: sub-tets sharing node 0)
+    for (int a = 0; a < 3; ++a) {
+      double sum = X[(int64_t)e[0] * 3 + a];
+      for (int q = 1; q < t.nnode; ++q) sum = sum + X[(int64_t)e[q] * 3 + a];
+      m.cell_center[(size_t)(c * 3 + a)] = sum / t.nnode;
     }
-    double cr0 = dy[0] * dz[1] - dz[0] * dy[1];
-    double cr1 = dz[0] * dx[1] - dx[0] * dz[1];
-    double cr2 = dx[0] * dy[1] - dy[0] * dx[1];
     double vol = 0.0;
-    vol += (cr0 * dx[2] + cr1 * dy[2] + cr2 * dz[2]) / 6.0;
+    for (int it = 0; it < t.ntet; ++it) {
+      double dx[3], dy[3], dz[3];
+      for (int in = 0; in < 3; ++in) {
+        dx[in] = X[e[t.vn[it][in]] * 3 + 0] - X[e[0] * 3 + 0];
+        dy[in] = X[e[t.vn[it][in]] * 3 + 1] - X[e[0] * 3 + 1];
+        dz[in] = X[e[t.vn[it][in]] * 3 + 2] - X[e[0] * 3 + 2];
+      }
+      double cr0 = dy[0] * dz[1] - dz[0] * dy[1];
+      double cr1 = dz[0] * dx[1] - dx[0] * dz[1];
+      double cr2 = dx[0] * dy[1] - dy[0] * dx[1];
+      vol += (cr0 * dx[2] + cr1 * dy[2] + cr2 * dz[2]) / 6.0;
+    }
     m.volume[(size_t)c] = vol;
-    if (!(vol > 0.0)) return fail(FK_EINVAL, "cell with non-positive volume (inverted tet)");
+    if (!(vol > 0.0)) return fail(FK_EINVAL, "cell with non-positive volume (inverted element)");
   }
   return FK_OK;
 }
@@ -377,9 +435,10 @@ void build_tangents(fkm_mesh &m) {
   for (int64_t f = 0; f < m.nf; ++f) {
     int64_t c = m.fc2el[(size_t)(f * 2)];
     int j = m.fc2elfc[(size_t)(f * 2)];
+    const ElemType &t = etype(m.nnode[(size_t)c]);
     double x[2], y[2], z[2];
     for (int k = 0; k < 2; ++k) {
-      int64_t nd = m.el2nd[(size_t)(c * 4 + kFaceNode[j][k])];
+      int64_t nd = m.el2nd[(size_t)(c * kNd + t.fn[j][k])];
       x[k] = X[nd * 3 + 0];
       y[k] = X[nd * 3 + 1];
       z[k] = X[nd * 3 + 2];
@@ -401,9 +460,9 @@ int finish_mesh(fkm_mesh &m, int order) {
   if (rc) return rc;
   // boundary faces must carry a code
   for (int64_t c = 0; c < m.nc; ++c)
-    for (int j = 0; j < 4; ++j)
-      if (m.nb[(size_t)(c * 4 + j)] < 0 && m.elbc[(size_t)(c * 4 + j)] == 0)
-        m.elbc[(size_t)(c * 4 + j)] = 2;  // untagged boundary faces: zero-gradient outlet
+    for (int j = 0; j < nface_of(m, c); ++j)
+      if (m.nb[(size_t)(c * kFc + j)] < 0 && m.elbc[(size_t)(c * kFc + j)] == 0)
+        m.elbc[(size_t)(c * kFc + j)] = 2;  // untagged boundary faces: zero-gradient outlet
   if (order == 1) apply_order(m, rcm_order(m));
   else if (order == 2) apply_order(m, morton_order(m));
   return build_faces_and_geometry(m);
@@ -415,8 +474,89 @@ extern "C" {
 
 const char *fkm_last_error(void) { return g_err.c_str(); }
 
-int fkm_box(int nx, int ny, int nz, const double lo[3], const double hi[3], double jitter, uint64_t seed,
-            int periodic_mask, const int bc[6], int order, fkm_mesh **out) {
+}  // extern "C"
+
+namespace {
+
+// Elements of one hex of the box (hex vertex v = a + 2b + 4c of the lattice
+// offsets; 8 = the hex's centre node).
+struct HexElems {
+  int n = 0;
+  int nnode[12];
+  int v[12][8];
+  void add(int nn, const int *vv) {
+    nnode[n] = nn;
+    for (int q = 0; q < nn; ++q) v[n][q] = vv[q];
+    ++n;
+  }
+};
+
+// Hex face s (0/1: x = 0/1, 2/3: y, 4/5: z): its four vertices as a pyramid
+// base (b0, b1, b2, b3 with cyclic order b0 b1 b3 b2 and (b1 - b0) x (b2 - b0)
+// pointing into the hex), and its two triangles along the diagonal of the
+// 6-tet split (the diagonal through hex vertex 0 or 7).
+const int kHexFaceBase[6][4] = {{0, 2, 4, 6}, {1, 5, 3, 7}, {0, 4, 1, 5}, {2, 3, 6, 7}, {0, 1, 2, 3}, {4, 6, 5, 7}};
+const int kHexFaceTri[6][2][3] = {{{0, 2, 6}, {0, 6, 4}}, {{1, 3, 7}, {1, 7, 5}}, {{0, 1, 5}, {0, 5, 4}},
+                                  {{2, 3, 7}, {2, 7, 6}}, {{0, 1, 3}, {0, 3, 2}}, {{4, 5, 7}, {4, 7, 6}}};
+
+// Face shape a hex of kind `kind` presents on its face s: 0 triangles (along
+// the 6-tet diagonal), 1 quadrilateral, -1 adapts to its neighbour.
+int face_shape(int kind, int s) {
+  switch (kind) {
+    case 0: return 0;
+    case 1: return 1;
+    case 2: return s >= 4 ? 0 : 1;
+    default: return -1;
+  }
+}
+
+// lattice offset (in half spacings) of hex vertex v (8 = centre) along axis a
+inline int half_off(int v, int a) { return v == 8 ? 1 : 2 * ((v >> a) & 1); }
+
+void hex_elems(int kind, const int shape[6], HexElems &h) {
+  // 6 tets per hex sharing the diagonal 0-7
+  static const int kTet6[6][4] = {{0, 1, 3, 7}, {0, 1, 5, 7}, {0, 2, 3, 7}, {0, 2, 6, 7}, {0, 4, 5, 7}, {0, 4, 6, 7}};
+  auto add_tet = [&](int a, int b, int c, int d) {
+    int v[4] = {a, b, c, d};
+    // orient positively on the undistorted lattice
+    double e[3][3];
+    for (int q = 0; q < 3; ++q)
+      for (int x = 0; x < 3; ++x) e[q][x] = half_off(v[q + 1], x) - half_off(v[0], x);
+    double det = e[0][0] * (e[1][1] * e[2][2] - e[1][2] * e[2][1]) - e[0][1] * (e[1][0] * e[2][2] - e[1][2] * e[2][0]) +
+                 e[0][2] * (e[1][0] * e[2][1] - e[1][1] * e[2][0]);
+    if (det < 0) std::swap(v[1], v[2]);
+    h.add(4, v);
+  };
+  if (kind == 0) {
+    for (int t = 0; t < 6; ++t) add_tet(kTet6[t][0], kTet6[t][1], kTet6[t][2], kTet6[t][3]);
+  } else if (kind == 1) {
+    const int v[8] = {0, 1, 2, 3, 4, 5, 6, 7};
+    h.add(8, v);
+  } else if (kind == 2) {
+    // two prisms along z, split by the plane through vertices 0, 3, 4, 7
+    const int p1[6] = {0, 1, 3, 4, 5, 7}, p2[6] = {0, 3, 2, 4, 7, 6};
+    h.add(6, p1);
+    h.add(6, p2);
+  } else {
+    // centre node 8: a pyramid on every quadrilateral face, two tets on every
+    // triangulated one
+    for (int sf = 0; sf < 6; ++sf) {
+      if (shape[sf] == 1) {
+        const int v[5] = {kHexFaceBase[sf][0], kHexFaceBase[sf][1], kHexFaceBase[sf][2], kHexFaceBase[sf][3], 8};
+        h.add(5, v);
+      } else {
+        for (int t = 0; t < 2; ++t) add_tet(kHexFaceTri[sf][t][0], kHexFaceTri[sf][t][1], kHexFaceTri[sf][t][2], 8);
+      }
+    }
+  }
+}
+
+}  // namespace
+
+namespace {
+
+int box_impl(int nx, int ny, int nz, const double lo[3], const double hi[3], double jitter, uint64_t seed,
+             int periodic_mask, const int bc[6], int order, const int32_t *kinds, fkm_mesh **out) {
   if (!out || !lo || !hi || !bc) return fail(FK_EINVAL, "null argument");
   *out = nullptr;
   if (nx < 1 || ny < 1 || nz < 1) return fail(FK_EINVAL, "box needs at least one hex per axis");
@@ -430,13 +570,56 @@ int fkm_box(int nx, int ny, int nz, const double lo[3], const double hi[3], doub
   for (int s = 0; s < 6; ++s)
     if (!(periodic_mask >> (s / 2) & 1) && (bc[s] < 1 || bc[s] > 5))
       return fail(FK_EINVAL, "boundary codes must be 1..5");
-  int64_t nc = (int64_t)nx * ny * nz * 6;
-  if (nc > INT32_MAX / 4) return fail(FK_ERANGE, "mesh too large for int32 cell ids");
+  const int64_t nhex = (int64_t)nx * ny * nz;
+  auto kind_of = [&](int64_t i, int64_t j, int64_t k) -> int {
+    return kinds ? kinds[(size_t)(i + nx * (j + (int64_t)ny * k))] : 0;
+  };
+  // per hex and face: the shape it presents (conformity between neighbours)
+  std::vector<int8_t> shape;
+  int64_t nc = 0, ncentre = 0;
+  if (kinds) {
+    if (periodic_mask) return fail(FK_EINVAL, "mixed element kinds need a box without periodic axes");
+    try {
+      shape.assign((size_t)(nhex * 6), 1);
+    } catch (const std::bad_alloc &) {
+      return fail(FK_ENOMEM, "out of host memory");
+    }
+    for (int64_t k = 0; k < nz; ++k)
+      for (int64_t j = 0; j < ny; ++j)
+        for (int64_t i = 0; i < nx; ++i) {
+          const int kd = kind_of(i, j, k);
+          if (kd < 0 || kd > 3) return fail(FK_EINVAL, "element kinds must be 0..3");
+          const int64_t I[3] = {i, j, k};
+          for (int sf = 0; sf < 6; ++sf) {
+            const int a = sf / 2, dir = sf % 2 ? 1 : -1;
+            int64_t J[3] = {I[0], I[1], I[2]};
+            J[a] += dir;
+            const bool inside = J[a] >= 0 && J[a] < N[a];
+            int mine = face_shape(kd, sf);
+            const int theirs = inside ? face_shape(kind_of(J[0], J[1], J[2]), sf ^ 1) : -1;
+            if (mine < 0) mine = theirs < 0 ? 1 : theirs;
+            else if (theirs >= 0 && theirs != mine)
+              return fail(FK_EINVAL, "element kinds do not conform: a quadrilateral face meets triangles "
+                                     "(put kind 3 between them)");
+            shape[(size_t)((i + nx * (j + (int64_t)ny * k)) * 6 + sf)] = (int8_t)mine;
+          }
+          HexElems h;
+          int sh[6];
+          for (int sf = 0; sf < 6; ++sf) sh[sf] = shape[(size_t)((i + nx * (j + (int64_t)ny * k)) * 6 + sf)];
+          hex_elems(kd, sh, h);
+          nc += h.n;
+          ncentre += kd == 3;
+        }
+  } else {
+    nc = nhex * 6;
+  }
+  if (nc > INT32_MAX / kFc) return fail(FK_ERANGE, "mesh too large for int32 cell ids");
   fkm_mesh *m = new (std::nothrow) fkm_mesh;
   if (!m) return fail(FK_ENOMEM, "out of host memory");
   try {
     const int64_t px = nx + 1, py = ny + 1, pz = nz + 1;
-    m->nn = px * py * pz;
+    const int64_t nlat = px * py * pz;
+    m->nn = nlat + ncentre;
     m->nc = nc;
     m->nodes.resize((size_t)(m->nn * 3));
     m->periodic = periodic_mask != 0;
@@ -472,54 +655,65 @@ int fkm_box(int nx, int ny, int nz, const double lo[3], const double hi[3], doub
             m->nodes[(size_t)(id * 3 + a)] = x;
           }
         }
-    // 6 tets per hex sharing the diagonal 0-7 (hex vertex v = a + 2b + 4c)
-    static const int kTet[6][4] = {{0, 1, 3, 7}, {0, 1, 5, 7}, {0, 2, 3, 7}, {0, 2, 6, 7}, {0, 4, 5, 7}, {0, 4, 6, 7}};
-    m->el2nd.resize((size_t)(nc * 4));
-    m->elbc.assign((size_t)(nc * 4), 0);
-    if (periodic_mask) m->elshift.assign((size_t)(nc * 12), 0.0);
-    int64_t c = 0;
+    m->el2nd.assign((size_t)(nc * kNd), -1);
+    m->nnode.assign((size_t)nc, 0);
+    m->elbc.assign((size_t)(nc * kFc), 0);
+    if (periodic_mask) m->elshift.assign((size_t)(nc * kFc * 3), 0.0);
+    int64_t c = 0, centre = nlat;
     for (int64_t k = 0; k < nz; ++k)
       for (int64_t j = 0; j < ny; ++j)
-        for (int64_t i = 0; i < nx; ++i)
-          for (int t = 0; t < 6; ++t, ++c) {
-            int v[4];
-            for (int q = 0; q < 4; ++q) v[q] = kTet[t][q];
-            // orient positively on the undistorted lattice
-            auto off = [](int vv, int a) { return (vv >> a) & 1; };
-            double d[3][3];
-            for (int q = 0; q < 3; ++q)
-              for (int a = 0; a < 3; ++a) d[q][a] = off(v[q + 1], a) - off(v[0], a);
-            double det = d[0][0] * (d[1][1] * d[2][2] - d[1][2] * d[2][1]) - d[0][1] * (d[1][0] * d[2][2] - d[1][2] * d[2][0]) +
-                         d[0][2] * (d[1][0] * d[2][1] - d[1][1] * d[2][0]);
-            if (det < 0) std::swap(v[1], v[2]);
-            for (int q = 0; q < 4; ++q) {
-              int64_t ii = i + off(v[q], 0), jj = j + off(v[q], 1), kk = k + off(v[q], 2);
-              m->el2nd[(size_t)(c * 4 + q)] = (int32_t)(ii + px * (jj + py * kk));
+        for (int64_t i = 0; i < nx; ++i) {
+          const int64_t hx = i + nx * (j + (int64_t)ny * k);
+          const int kd = kind_of(i, j, k);
+          int sh[6] = {1, 1, 1, 1, 1, 1};
+          if (kinds)
+            for (int sf = 0; sf < 6; ++sf) sh[sf] = shape[(size_t)(hx * 6 + sf)];
+          HexElems he;
+          hex_elems(kd, sh, he);
+          int64_t gid[9];
+          for (int v = 0; v < 8; ++v) gid[v] = (i + (v & 1)) + px * ((j + (v >> 1 & 1)) + py * (k + (v >> 2 & 1)));
+          if (kd == 3) {
+            gid[8] = centre++;
+            for (int a = 0; a < 3; ++a) {
+              double sum = 0;
+              for (int v = 0; v < 8; ++v) sum += m->nodes[(size_t)(gid[v] * 3 + a)];
+              m->nodes[(size_t)(gid[8] * 3 + a)] = sum / 8;
             }
+          }
+          for (int e = 0; e < he.n; ++e, ++c) {
+            const int nn = he.nnode[e];
+            m->nnode[(size_t)c] = nn;
+            for (int q = 0; q < nn; ++q) m->el2nd[(size_t)(c * kNd + q)] = (int32_t)gid[he.v[e][q]];
             // boundary code / periodic shift per local face from lattice positions
-            for (int fj = 0; fj < 4; ++fj) {
-              int64_t L[3][3];
-              for (int q = 0; q < 3; ++q) {
-                int vv = v[kFaceNode[fj][q]];
-                L[q][0] = i + off(vv, 0);
-                L[q][1] = j + off(vv, 1);
-                L[q][2] = k + off(vv, 2);
+            const ElemType &t = etype(nn);
+            for (int fj = 0; fj < t.nface; ++fj) {
+              int64_t L[4][3];
+              const int kq = t.fnn[fj];
+              for (int q = 0; q < kq; ++q) {
+                const int vv = he.v[e][t.fn[fj][q]];
+                L[q][0] = 2 * i + half_off(vv, 0);
+                L[q][1] = 2 * j + half_off(vv, 1);
+                L[q][2] = 2 * k + half_off(vv, 2);
               }
               for (int a = 0; a < 3; ++a) {
-                bool all0 = L[0][a] == 0 && L[1][a] == 0 && L[2][a] == 0;
-                bool allN = L[0][a] == N[a] && L[1][a] == N[a] && L[2][a] == N[a];
+                bool all0 = true, allN = true;
+                for (int q = 0; q < kq; ++q) {
+                  all0 = all0 && L[q][a] == 0;
+                  allN = allN && L[q][a] == 2 * (int64_t)N[a];
+                }
                 if (periodic_mask >> a & 1) {
                   // face on the upper periodic plane: the neighbour sits at the
                   // lower end, its copy of the face is translated by -(hi-lo)
-                  if (allN) m->elshift[(size_t)(c * 12 + fj * 3 + a)] = hi[a] - lo[a];
-                  if (all0) m->elshift[(size_t)(c * 12 + fj * 3 + a)] = -(hi[a] - lo[a]);
+                  if (allN) m->elshift[(size_t)((c * kFc + fj) * 3 + a)] = hi[a] - lo[a];
+                  if (all0) m->elshift[(size_t)((c * kFc + fj) * 3 + a)] = -(hi[a] - lo[a]);
                 } else {
-                  if (all0) m->elbc[(size_t)(c * 4 + fj)] = bc[2 * a];
-                  if (allN) m->elbc[(size_t)(c * 4 + fj)] = bc[2 * a + 1];
+                  if (all0) m->elbc[(size_t)(c * kFc + fj)] = bc[2 * a];
+                  if (allN) m->elbc[(size_t)(c * kFc + fj)] = bc[2 * a + 1];
                 }
               }
             }
           }
+        }
     int rc = finish_mesh(*m, order);
     if (rc) {
       delete m;
@@ -531,6 +725,21 @@ int fkm_box(int nx, int ny, int nz, const double lo[3], const double hi[3], doub
   }
   *out = m;
   return FK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fkm_box(int nx, int ny, int nz, const double lo[3], const double hi[3], double jitter, uint64_t seed,
+            int periodic_mask, const int bc[6], int order, fkm_mesh **out) {
+  return box_impl(nx, ny, nz, lo, hi, jitter, seed, periodic_mask, bc, order, nullptr, out);
+}
+
+int fkm_box_kinds(int nx, int ny, int nz, const double lo[3], const double hi[3], double jitter, uint64_t seed,
+                  const int bc[6], int order, const int32_t *kinds, fkm_mesh **out) {
+  if (!kinds) return fail(FK_EINVAL, "null kinds");
+  return box_impl(nx, ny, nz, lo, hi, jitter, seed, 0, bc, order, kinds, out);
 }
 
 int fkm_read_neu(const char *path, double scale, int order, fkm_mesh **out) {
@@ -563,14 +772,15 @@ int fkm_read_neu(const char *path, double scale, int order, fkm_mesh **out) {
   if (!next_section("NUMNP")) return bail(FK_EIO, "neu: no NUMNP header");
   if (std::fscanf(fp, "%ld %ld %ld %ld %ld %ld", &ndnum, &elnum, &ngrps, &bcnum, &ndfcd, &ndfvl) != 6)
     return bail(FK_EIO, "neu: bad size line");
-  if (ndnum <= 0 || elnum <= 0 || elnum > INT32_MAX / 4) return bail(FK_EINVAL, "neu: bad node/element counts");
+  if (ndnum <= 0 || elnum <= 0 || elnum > INT32_MAX / kFc) return bail(FK_EINVAL, "neu: bad node/element counts");
   if (!next_section("NODAL COORDINATES")) return bail(FK_EIO, "neu: no NODAL COORDINATES section");
   try {
     m->nn = ndnum;
     m->nc = elnum;
     m->nodes.resize((size_t)(ndnum * 3));
-    m->el2nd.resize((size_t)(elnum * 4));
-    m->elbc.assign((size_t)(elnum * 4), 0);
+    m->el2nd.assign((size_t)(elnum * kNd), -1);
+    m->nnode.assign((size_t)elnum, 0);
+    m->elbc.assign((size_t)(elnum * kFc), 0);
   } catch (const std::bad_alloc &) {
     return bail(FK_ENOMEM, "neu: out of memory");
   }
@@ -584,17 +794,23 @@ int fkm_read_neu(const char *path, double scale, int order, fkm_mesh **out) {
     m->nodes[(size_t)((id - 1) * 3 + 2)] = z * scale;
   }
   if (!next_section("ELEMENTS/CELLS")) return bail(FK_EIO, "neu: no ELEMENTS/CELLS section");
+  // element type by node count, as translator.cu:122-293: 8 hex, 6 prism,
+  // 5 pyramid, 4 tet, nodes in the element's Gambit order
   for (long i = 0; i < elnum; ++i) {
-    long el, typ, nnd, c[4];
+    long el, typ, nnd, c[kNd];
     if (std::fscanf(fp, "%ld %ld %ld", &el, &typ, &nnd) != 3 || el < 1 || el > elnum)
       return bail(FK_EIO, "neu: bad element line");
-    if (nnd != 4) return bail(FK_EINVAL, "neu: only linear tetrahedra are supported");
-    if (std::fscanf(fp, "%ld %ld %ld %ld", &c[0], &c[1], &c[2], &c[3]) != 4) return bail(FK_EIO, "neu: bad element nodes");
-    for (int q = 0; q < 4; ++q) {
+    if (!valid_nnode(nnd)) return bail(FK_EINVAL, "neu: elements must have 4, 5, 6 or 8 nodes (linear tet, pyramid, prism, hex)");
+    for (int q = 0; q < nnd; ++q)
+      if (std::fscanf(fp, "%ld", &c[q]) != 1) return bail(FK_EIO, "neu: bad element nodes");
+    m->nnode[(size_t)(el - 1)] = (int32_t)nnd;
+    for (int q = 0; q < nnd; ++q) {
       if (c[q] < 1 || c[q] > ndnum) return bail(FK_EINVAL, "neu: node id out of range");
-      m->el2nd[(size_t)((el - 1) * 4 + q)] = (int32_t)(c[q] - 1);
+      m->el2nd[(size_t)((el - 1) * kNd + q)] = (int32_t)(c[q] - 1);
     }
   }
+  for (long i = 0; i < elnum; ++i)
+    if (m->nnode[(size_t)i] == 0) return bail(FK_EIO, "neu: element missing from ELEMENTS/CELLS");
   for (long b = 0; b < bcnum; ++b) {
     if (!next_section("BOUNDARY CONDITIONS")) return bail(FK_EIO, "neu: missing BOUNDARY CONDITIONS section");
     char name[256];
@@ -605,8 +821,9 @@ int fkm_read_neu(const char *path, double scale, int order, fkm_mesh **out) {
     for (long i = 0; i < nentry; ++i) {
       long el, typ, face;
       if (std::fscanf(fp, "%ld %ld %ld", &el, &typ, &face) != 3) return bail(FK_EIO, "neu: bad boundary entry");
-      if (el < 1 || el > elnum || face < 1 || face > 4) return bail(FK_EINVAL, "neu: boundary entry out of range");
-      if (bcv) m->elbc[(size_t)((el - 1) * 4 + face - 1)] = bcv;
+      if (el < 1 || el > elnum || face < 1 || face > etype(m->nnode[(size_t)(el - 1)]).nface)
+        return bail(FK_EINVAL, "neu: boundary entry out of range");
+      if (bcv) m->elbc[(size_t)((el - 1) * kFc + face - 1)] = bcv;
     }
   }
   std::fclose(fp);
@@ -632,8 +849,8 @@ int fkm_write_neu(const fkm_mesh *m, const char *path) {
   static const char *kName[6] = {"", "symmetry", "outlet", "inlet", "slip", "wall"};
   int64_t count[6] = {0, 0, 0, 0, 0, 0};
   for (int64_t c = 0; c < m->nc; ++c)
-    for (int j = 0; j < 4; ++j)
-      if (m->nb[(size_t)(c * 4 + j)] < 0) count[m->elbc[(size_t)(c * 4 + j)]]++;
+    for (int j = 0; j < nface_of(*m, c); ++j)
+      if (m->nb[(size_t)(c * kFc + j)] < 0) count[m->elbc[(size_t)(c * kFc + j)]]++;
   int nsets = 0;
   for (int b = 1; b <= 5; ++b) nsets += count[b] > 0;
   std::fprintf(fp, "CONTROL INFO         1.2.1\r\n** GAMBIT NEUTRAL FILE\r\nfooska\r\nPROGRAM:  fooska\r\n\r\n");
@@ -644,9 +861,14 @@ int fkm_write_neu(const fkm_mesh *m, const char *path) {
     std::fprintf(fp, "%10lld %.17e %.17e %.17e\r\n", (long long)(i + 1), m->nodes[(size_t)(i * 3)], m->nodes[(size_t)(i * 3 + 1)],
                  m->nodes[(size_t)(i * 3 + 2)]);
   std::fprintf(fp, "ENDOFSECTION\r\nELEMENTS/CELLS       1.2.1\r\n");
-  for (int64_t c = 0; c < m->nc; ++c)
-    std::fprintf(fp, "%8lld %2d %2d %8d %8d %8d %8d\r\n", (long long)(c + 1), 6, 4, m->el2nd[(size_t)(c * 4)] + 1,
-                 m->el2nd[(size_t)(c * 4 + 1)] + 1, m->el2nd[(size_t)(c * 4 + 2)] + 1, m->el2nd[(size_t)(c * 4 + 3)] + 1);
+  for (int64_t c = 0; c < m->nc; ++c) {
+    // Gambit element types: 4 brick, 5 wedge, 6 tetrahedron, 7 pyramid
+    const int nn = m->nnode[(size_t)c];
+    const int gtype = nn == 8 ? 4 : nn == 6 ? 5 : nn == 4 ? 6 : 7;
+    std::fprintf(fp, "%8lld %2d %2d", (long long)(c + 1), gtype, nn);
+    for (int q = 0; q < nn; ++q) std::fprintf(fp, " %8d", m->el2nd[(size_t)(c * kNd + q)] + 1);
+    std::fprintf(fp, "\r\n");
+  }
   std::fprintf(fp, "ENDOFSECTION\r\nELEMENT GROUP        1.2.1\r\nGROUP: 1 ELEMENTS: %lld MATERIAL: 2 NFLAGS: 1\r\nfluid\r\n0\r\n",
                (long long)m->nc);
   for (int64_t c = 0; c < m->nc; ++c) std::fprintf(fp, "%8lld%s", (long long)(c + 1), (c % 10 == 9 || c == m->nc - 1) ? "\r\n" : "");
@@ -654,10 +876,13 @@ int fkm_write_neu(const fkm_mesh *m, const char *path) {
   for (int b = 1; b <= 5; ++b) {
     if (!count[b]) continue;
     std::fprintf(fp, "BOUNDARY CONDITIONS  1.2.1\r\n%32s%8d%8lld%8d%8d\r\n", kName[b], 1, (long long)count[b], 0, kCode[b]);
-    for (int64_t c = 0; c < m->nc; ++c)
-      for (int j = 0; j < 4; ++j)
-        if (m->nb[(size_t)(c * 4 + j)] < 0 && m->elbc[(size_t)(c * 4 + j)] == b)
-          std::fprintf(fp, "%10lld %5d %5d\r\n", (long long)(c + 1), 6, j + 1);
+    for (int64_t c = 0; c < m->nc; ++c) {
+      const int nn = m->nnode[(size_t)c];
+      const int gtype = nn == 8 ? 4 : nn == 6 ? 5 : nn == 4 ? 6 : 7;
+      for (int j = 0; j < nface_of(*m, c); ++j)
+        if (m->nb[(size_t)(c * kFc + j)] < 0 && m->elbc[(size_t)(c * kFc + j)] == b)
+          std::fprintf(fp, "%10lld %5d %5d\r\n", (long long)(c + 1), gtype, j + 1);
+    }
     std::fprintf(fp, "ENDOFSECTION\r\n");
   }
   int bad = std::ferror(fp);
@@ -683,6 +908,7 @@ int fkm_array(const fkm_mesh *mc, const char *name, void **ptr) {
   if (s == "fc2el") *ptr = m->fc2el.data();
   else if (s == "bc") *ptr = m->bc.data();
   else if (s == "el2nd") *ptr = m->el2nd.data();
+  else if (s == "nnode") *ptr = m->nnode.data();
   else if (s == "n") *ptr = m->n.data();
   else if (s == "nt1") { build_tangents(*m); *ptr = m->nt1.data(); }
   else if (s == "nt2") { build_tangents(*m); *ptr = m->nt2.data(); }

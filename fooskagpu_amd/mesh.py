@@ -1,7 +1,10 @@
 """Host mesh: box generator and Gambit neutral file reader/writer
 (libfooska_mesh.so).  Arrays follow the reference MESH semantics
 (mulitphase Solver/doublePrecision/strdata.h:100-115): fc2el, boundCond,
-n, area, fcCenter, cellCenter, volume, plus the periodic face shift."""
+n, area, fcCenter, cellCenter, volume, plus the periodic face shift.
+Cells are linear tetrahedra, pyramids, prisms or hexahedra (4, 5, 6 or 8
+nodes, el2nd[cell][8] as the reference's elNd2GlbNd, -1 past the element's
+nodes)."""
 import ctypes
 
 import numpy as np
@@ -11,8 +14,14 @@ from . import _lib
 BC_SYMMETRY, BC_OUTLET, BC_INLET, BC_SLIP, BC_WALL = 1, 2, 3, 4, 5
 
 
+# hex element kinds of Mesh.box(kinds=...)
+KIND_TETS, KIND_HEX, KIND_PRISMS, KIND_SPLIT = 0, 1, 2, 3
+# faces per element, by node count
+FACES_OF = {4: 4, 5: 5, 6: 5, 8: 6}
+
+
 class Mesh:
-    """Tetrahedral mesh; every array is an owned numpy copy."""
+    """Unstructured mesh; every array is an owned numpy copy."""
 
     def __init__(self, handle):
         L = _lib.mesh_lib()
@@ -38,7 +47,8 @@ class Mesh:
         self.shift = arr("shift", np.float64, (F, 3))
         self.cell_center = arr("cell_center", np.float64, (C, 3))
         self.volume = arr("volume", np.float64, (C,))
-        self.el2nd = arr("el2nd", np.int32, (C, 4))
+        self.el2nd = arr("el2nd", np.int32, (C, 8))
+        self.nnode = arr("nnode", np.int32, (C,))
         self.nodes = arr("nodes", np.float64, (self.n_nodes, 3))
         self.periodic = bool(np.any(self.shift != 0))
 
@@ -67,10 +77,20 @@ class Mesh:
     # ---- construction ----
     ORDERS = {"none": 0, "rcm": 1, "morton": 2}
 
+    @property
+    def faces_per_cell(self):
+        """Faces of each cell (4 tet, 5 pyramid or prism, 6 hex)."""
+        return np.select([self.nnode == 8, self.nnode == 4], [6, 4], 5)
+
     @classmethod
     def box(cls, nx, ny, nz, lo=(0, 0, 0), hi=(1, 1, 1), jitter=0.0, seed=0, periodic=(False, False, False),
-            bc=(BC_OUTLET,) * 6, rcm=True, order=None):
-        """order: "rcm" (default), "morton" or "none" (rcm=False)."""
+            bc=(BC_OUTLET,) * 6, rcm=True, order=None, kinds=None):
+        """order: "rcm" (default), "morton" or "none" (rcm=False).
+        kinds: None (every hex split into 6 tets) or an int array [nz][ny][nx]
+        of KIND_TETS, KIND_HEX, KIND_PRISMS (two prisms along z) or
+        KIND_SPLIT (a centre node with a pyramid on each quadrilateral face
+        and two tets on each face that meets triangles); neighbouring hexes
+        must present the same face shape, KIND_SPLIT adapts to both."""
         L = _lib.mesh_lib()
         lo_a = np.ascontiguousarray(lo, dtype=np.float64)
         hi_a = np.ascontiguousarray(hi, dtype=np.float64)
@@ -78,9 +98,30 @@ class Mesh:
         mask = sum(1 << i for i, p in enumerate(periodic) if p)
         h = ctypes.c_void_p()
         o = cls.ORDERS[order] if order is not None else (1 if rcm else 0)
-        _lib.check_mesh(L.fkm_box(nx, ny, nz, _lib.dptr(lo_a), _lib.dptr(hi_a), jitter, seed, mask,
-                                  _lib.iptr(bc_a), o, ctypes.byref(h)))
+        if kinds is None:
+            _lib.check_mesh(L.fkm_box(nx, ny, nz, _lib.dptr(lo_a), _lib.dptr(hi_a), jitter, seed, mask,
+                                      _lib.iptr(bc_a), o, ctypes.byref(h)))
+        else:
+            if mask:
+                raise ValueError("mixed element kinds need a box without periodic axes")
+            k = np.ascontiguousarray(np.broadcast_to(np.asarray(kinds, dtype=np.int32), (nz, ny, nx)))
+            _lib.check_mesh(L.fkm_box_kinds(nx, ny, nz, _lib.dptr(lo_a), _lib.dptr(hi_a), jitter, seed,
+                                            _lib.iptr(bc_a), o, _lib.iptr(k), ctypes.byref(h)))
         return cls(h)
+
+    @classmethod
+    def mixed_box(cls, nx, ny, nz, lo=(0, 0, 0), hi=(1, 1, 1), jitter=0.0, seed=0, bc=(BC_OUTLET,) * 6, rcm=True,
+                  order=None):
+        """Every element type of the reference in one conforming box: five
+        slabs along x (nx >= 5) of tets | split hexes (pyramids + the tets
+        facing the tet slab) | hexes | split hexes (pyramids) | prisms."""
+        if nx < 5:
+            raise ValueError("mixed_box needs nx >= 5")
+        edges = np.linspace(0, nx, 6).round().astype(int)
+        slab = np.searchsorted(edges, np.arange(nx), side="right") - 1
+        kind_x = np.array([KIND_TETS, KIND_SPLIT, KIND_HEX, KIND_SPLIT, KIND_PRISMS])[slab]
+        kinds = np.broadcast_to(kind_x, (nz, ny, nx))
+        return cls.box(nx, ny, nz, lo, hi, jitter, seed, bc=bc, rcm=rcm, order=order, kinds=kinds)
 
     @classmethod
     def read_neu(cls, path, scale=1.0, rcm=True):

@@ -1,8 +1,10 @@
 /*
  * fooska_mesh.h -- host-side mesh and case library (C ABI).
  *
- * Builds the unstructured tetrahedral mesh the finite-volume step runs on and
- * returns it in the reference solver's MESH field semantics:
+ * Builds the unstructured mesh the finite-volume step runs on (linear
+ * tetrahedra, pyramids, prisms and hexahedra, the element types of the
+ * reference's translator.cu:122-293) and returns it in the reference solver's
+ * MESH field semantics:
  *   fc2el[nFcs][2]   owner, neighbour (-1 on boundary faces), owner < neighbour
  *   boundCond[nFcs]  0 interior, 1 symmetry, 2 outlet, 3 inlet, 4 slip, 5 wall
  *   n, nt1, nt2      [nFcs][3] unit normal (outward from the owner) and tangents
@@ -56,9 +58,22 @@ int fkm_box(int nx, int ny, int nz, const double lo[3], const double hi[3],
             double jitter, uint64_t seed, int periodic_mask, const int bc[6],
             int order, fkm_mesh **out);
 
-/* Gambit neutral file (tets only), node coordinates multiplied by scale;
- * order as for fkm_box.
- * BC tags 45/26/13/42/51 map to codes 1/2/3/4/5 as in translator.cu:299-349. */
+/* As fkm_box without periodic axes, with the element kind of every hex
+ * (kinds[nz][ny][nx]): 0 six tets sharing the hex diagonal, 1 one hexahedron,
+ * 2 two prisms along z (split through hex vertices 0, 3, 4, 7), 3 a centre
+ * node with a pyramid on every quadrilateral face and two tets on every face
+ * that meets triangles.  Neighbouring hexes must present the same face shape
+ * (kind 3 adapts to its neighbours); FK_EINVAL otherwise.  No reference
+ * counterpart (the reference reads such meshes from Gambit files). */
+int fkm_box_kinds(int nx, int ny, int nz, const double lo[3], const double hi[3],
+                  double jitter, uint64_t seed, const int bc[6], int order,
+                  const int32_t *kinds, fkm_mesh **out);
+
+/* Gambit neutral file, node coordinates multiplied by scale; order as for
+ * fkm_box.  Elements by node count as translator.cu:122-293: 4 tetrahedron,
+ * 5 pyramid, 6 prism, 8 hexahedron, in Gambit's node order.  BC tags
+ * 45/26/13/42/51 map to codes 1/2/3/4/5 and the element face number of a
+ * boundary entry is the local face + 1, as in translator.cu:295-349. */
 int fkm_read_neu(const char *path, double scale, int order, fkm_mesh **out);
 /* Writes the mesh (generation-order-independent) as a Gambit neutral file. */
 int fkm_write_neu(const fkm_mesh *m, const char *path);
@@ -70,7 +85,8 @@ int fkm_sizes(const fkm_mesh *m, int64_t *n_cells, int64_t *n_faces,
               int64_t *n_nodes, int64_t *n_bfaces);
 
 /* Borrowed pointer to an internal array (valid until fkm_free).  Names:
- *  int32:  "fc2el" [F][2], "bc" [F], "el2nd" [C][4]
+ *  int32:  "fc2el" [F][2], "bc" [F], "el2nd" [C][8] (-1 past the element's
+ *          nodes), "nnode" [C] (4, 5, 6 or 8)
  *  double: "n" "nt1" "nt2" "fc_center" "shift" [F][3], "area" [F],
  *          "cell_center" [C][3], "volume" [C], "nodes" [N][3] */
 int fkm_array(const fkm_mesh *m, const char *name, void **ptr);
