@@ -32,7 +32,7 @@ class FkParams(ctypes.Structure):
         "polishTol")] + [(n, ctypes.c_double * 2) for n in (
         "gamma", "pInf", "Cp", "eta", "satConst", "dynVisc", "Pr")] + [
         ("thermFntRlxFlag", ctypes.c_int32), ("thermRlxRate", ctypes.c_double),
-        ("inletP", ctypes.c_double), ("inletT", ctypes.c_double)]
+        ("inletP", ctypes.c_double), ("inletT", ctypes.c_double), ("pRef", ctypes.c_double)]
 
     @classmethod
     def from_dict(cls, d):
@@ -84,6 +84,9 @@ MESH_API = [
     ("fkm_last_error", ctypes.c_char_p, []),
     ("fkm_box", ctypes.c_int, [ctypes.c_int] * 3 + [c_double_p, c_double_p, ctypes.c_double, ctypes.c_uint64,
                                ctypes.c_int, c_int32_p, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
+    ("fkm_box_kinds", ctypes.c_int, [ctypes.c_int] * 3 + [c_double_p, c_double_p, ctypes.c_double, ctypes.c_uint64,
+                                                         c_int32_p, ctypes.c_int, c_int32_p,
+                                                         ctypes.POINTER(ctypes.c_void_p)]),
     ("fkm_read_neu", ctypes.c_int, [ctypes.c_char_p, ctypes.c_double, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]),
     ("fkm_write_neu", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
     ("fkm_free", None, [ctypes.c_void_p]),
@@ -119,6 +122,7 @@ HIP_API = [
     ("fk_cons_calc", ctypes.c_int, [_ctx, c_double_p]),
     ("fk_cons_calc_noalloc", ctypes.c_int, [_ctx, c_double_p]),
     ("fk_run_steps", ctypes.c_int, [_ctx, ctypes.c_int, c_double_p, c_double_p]),
+    ("fk_health", ctypes.c_int, [_ctx, ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_int64), ctypes.c_int]),
     ("fk_sync", ctypes.c_int, [_ctx]),
     ("fk_get_clock", ctypes.c_int, [_ctx, c_double_p, c_double_p, ctypes.POINTER(ctypes.c_int64)]),
     ("fk_set_clock", ctypes.c_int, [_ctx, ctypes.c_double, ctypes.c_double, ctypes.c_int64]),

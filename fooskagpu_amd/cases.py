@@ -74,8 +74,10 @@ def tgv(n=64, steps=200, seed=1, rcm=True):
     jittered +-10 % of the spacing, Mach 0.1, Re 1600 (U0 = rho0 = L = 1),
     Pr 0.71, WALE (Cw = 0.325) and the Fourier heat flux."""
     gamma, mach, re = 1.4, 0.1, 1600.0
+    p0 = 1.0 / (gamma * mach * mach)  # background pressure rho0 U0^2 / (gamma Ma^2)
+    # pRef = p0: the mixed path carries the FP32 face pressures as p - p0
     p = dict(BASE, model=_lib.MODEL_SINGLE, gamma=[gamma, gamma], pInf=[0.0, 0.0], Cp=[1006.0, 1006.0],
-             viscFlag=1, turbulence=1, heatFlux=1, dynVisc=[1.0 / re, 1.0 / re], Pr=[0.71, 0.71])
+             viscFlag=1, turbulence=1, heatFlux=1, dynVisc=[1.0 / re, 1.0 / re], Pr=[0.71, 0.71], pRef=p0, cfl=0.3)
     two_pi = 2 * math.pi
     m = Mesh.box(n, n, n, (0, 0, 0), (two_pi,) * 3, 0.10, seed, periodic=(True, True, True), rcm=rcm)
     U = m.init_tgv(gamma, 1.0, 1.0, mach)

@@ -23,6 +23,10 @@ struct Mat {
   T g[2], pi[2], cp[2], eta[2];
   T avfMin, chemLimit, mu[2], Pr0;
   T inP, inT;  // inlet ghost pressure and temperature
+  // Single-phase, FP32 (mixed) path only: the FP32 face pressures hold
+  // p - pr (fk_params pRef), the absolute pressure is formed where the flux
+  // needs it.  Never read on the FP64 path.
+  T pr;
   int woods, hllcToro;
 };
 
@@ -41,6 +45,7 @@ __host__ __device__ inline Mat<T> make_mat(const fk_params &p) {
   m.Pr0 = (T)p.Pr[0];
   m.inP = (T)p.inletP;
   m.inT = (T)p.inletT;
+  m.pr = (T)p.pRef;
   m.woods = p.woodsFlag;
   m.hllcToro = p.hllcEnergy;
   return m;
@@ -169,7 +174,7 @@ __device__ __forceinline__ void prims_f32(const Mat<double> &m, const double *__
     s[1] = (float)u;
     s[2] = (float)v;
     s[3] = (float)w;
-    s[4] = (float)p1_of(m, U[0], e);
+    s[4] = (float)(p1_of(m, U[0], e) - m.pr);  // deviation from pRef (exact in FP32)
   } else {
     const double a = clamp_avf(m, U[0]);
     const double ia = rcp_nr(a), ig = rcp_nr(1.0 - a);
@@ -218,6 +223,7 @@ __device__ __forceinline__ void ghost(const Mat<T> &m, int bc, const T n[3], T *
     if constexpr (NEQ == 5) {
       s[0] = rho_of(m, 0, P, Tt);
       s[4] = P;
+      if constexpr (sizeof(T) == 4) s[4] = P - m.pr;  // FP32 path: deviation from pRef
     } else {
       s[0] = (T)1.0 - m.avfMin;
       s[1] = rho_of(m, 0, P, Tt);
@@ -245,10 +251,20 @@ template <int NEQ, typename T>
 __device__ __forceinline__ void riemann(const Mat<T> &m, const T *L, const T *R, const T n[3], int is_wall,
                                         FaceRes<T, NEQ> &out) {
   if constexpr (NEQ == 5) {
+    // FP32 path: L[4], R[4] are deviations from the reference pressure pr.
+    // Pressure differences (S*) and the momentum flux use them as they are
+    // (sum_f pr n_f A_f = 0 over a closed cell, so dropping pr there is
+    // exact and avoids rounding a large constant in FP32); sound speeds and
+    // the energy flux take the absolute pressure.
+    T pa_L = L[4], pa_R = R[4];
+    if constexpr (sizeof(T) == 4) {
+      pa_L = L[4] + m.pr;
+      pa_R = R[4] + m.pr;
+    }
     T uL = L[1] * n[0] + L[2] * n[1] + L[3] * n[2];
     T uR = R[1] * n[0] + R[2] * n[1] + R[3] * n[2];
-    T cL = c_of(m, 0, L[0], L[4]);
-    T cR = c_of(m, 0, R[0], R[4]);
+    T cL = c_of(m, 0, L[0], pa_L);
+    T cR = c_of(m, 0, R[0], pa_R);
     T S0 = fmin(uL - cL, uR - cR);
     T S2 = fmax(uL + cL, uR + cR);
     T S1 = (R[4] - L[4] + L[0] * uL * (S0 - uL) - R[0] * uR * (S2 - uR)) / (L[0] * (S0 - uL) - R[0] * (S2 - uR));
@@ -258,21 +274,22 @@ __device__ __forceinline__ void riemann(const Mat<T> &m, const T *L, const T *R,
     const T *s = left ? L : R;
     const T un = left ? uL : uR;
     const T SK = left ? S0 : S2;
+    const T ps = left ? pa_L : pa_R;  // absolute upwind pressure
     T ke = (T)0.5 * (s[1] * s[1] + s[2] * s[2] + s[3] * s[3]);
-    T e = e1_of(m, s[4], s[0]);
+    T e = e1_of(m, ps, s[0]);
     T H[5] = {0, 0, 0, 0, 0};
     if (star) {
       T c0 = s[0], c1 = s[0] * s[1], c2 = s[0] * s[2], c3 = s[0] * s[3], c4 = s[0] * (e + ke);
       T v0 = s[0] * (SK - un) / (SK - S1);
       T vg0 = s[1] + (S1 - un) * n[0], vg1 = s[2] + (S1 - un) * n[1], vg2 = s[3] + (S1 - un) * n[2];
-      T v4 = v0 * (e + ke + (S1 - un) * (S1 + s[4] / (s[0] * (SK - un))));
+      T v4 = v0 * (e + ke + (S1 - un) * (S1 + ps / (s[0] * (SK - un))));
       H[0] = SK * (v0 - c0);
       H[1] = SK * (v0 * vg0 - c1);
       H[2] = SK * (v0 * vg1 - c2);
       H[3] = SK * (v0 * vg2 - c3);
       H[4] = SK * (v4 - c4);
     }
-    T h = s[0] * (e + ke) + s[4];
+    T h = s[0] * (e + ke) + ps;
     T F0 = s[1] * s[0] * n[0] + s[2] * s[0] * n[1] + s[3] * s[0] * n[2];
     T F1 = (s[0] * s[1] * s[1] + s[4]) * n[0] + s[0] * s[1] * s[2] * n[1] + s[0] * s[1] * s[3] * n[2];
     T F2 = s[0] * s[2] * s[1] * n[0] + (s[0] * s[2] * s[2] + s[4]) * n[1] + s[0] * s[2] * s[3] * n[2];
@@ -395,7 +412,9 @@ template <int NEQ, typename T>
 __device__ __forceinline__ T signal(const Mat<T> &m, const T *W, const T n[3]) {
   if constexpr (NEQ == 5) {
     T un = W[1] * n[0] + W[2] * n[1] + W[3] * n[2];
-    return fabs(un) + c_of(m, 0, W[0], W[4]);
+    T pa = W[4];
+    if constexpr (sizeof(T) == 4) pa = W[4] + m.pr;  // FP32 path: deviation from pRef
+    return fabs(un) + c_of(m, 0, W[0], pa);
   } else {
     T avf = W[0], rL = W[1], rG = W[2];
     T rm = avf * rL + (1 - avf) * rG;

@@ -62,6 +62,7 @@ struct fk_ctx {
   fk_params p{};
   int neq = 5;
   int64_t nc = 0, nf = 0;
+  int ns = 4;  // face slots per cell: 4 (every cell a tetrahedron) or 6
   bool have_mesh = false, have_state = false, debug = false, timing = false;
   // mesh
   int32_t *link = nullptr;  // [4C] slot links (see fk_kernels.hip)
@@ -73,6 +74,9 @@ struct fk_ctx {
   int ia = 0, iu0 = 0;
   void *face = nullptr, *gvel = nullptr;
   double *rhs = nullptr, *dtc = nullptr, *red = nullptr, *dscal = nullptr;
+  // state health found by pass G: (step << 32) | cell, the smallest of the
+  // cells with a non-positive density or a non-finite primitive; ~0 if none
+  unsigned long long *health = nullptr;
   double *dgrad = nullptr, *dtheta = nullptr;
   // relaxation counters: [0, 5) of the last relaxation (n_chem, Newton
   // iterations, Tsat iterations, fallbacks, warm-started cells; slot 5 is the

@@ -110,8 +110,14 @@ __device__ __forceinline__ V4<T> ld4(const void *p, int64_t i) {
   return reinterpret_cast<const V4<T> *>(p)[i];
 }
 
+// Slots per cell: NS = 4 on meshes of tetrahedra, 6 on meshes with other
+// elements (pyramids and prisms have 5 faces, hexahedra 6); a cell's slots past
+// its last face hold the link kNoFace and zero geometry.
+constexpr int kNoFace = -8;
+
 // slot link -> neighbour cell (or -bc) and owner flag
-__device__ __forceinline__ int link_nb(int lk) { return lk >= 0 ? (lk >> 3) : lk; }
+template <int NS>
+__device__ __forceinline__ int link_nb(int lk) { return lk >= 0 ? (lk >> 1) / NS : lk; }
 __device__ __forceinline__ bool link_own(int lk) { return lk < 0 || (lk & 1); }
 template <typename T>
 __device__ __forceinline__ int link_of(T w) {
@@ -122,7 +128,7 @@ __device__ __forceinline__ int link_of(T w) {
 // Pass G: first-order states, Green-Gauss gradient, neighbour min/max,
 // limiter and MUSCL face states (computePrimVec.cu:38-80,
 // computeGrad.cu:51-91, computeFaceValues.cu:29-113).  Every cell converts
-// its own state and its 4 neighbours' (cons2prim, FP64) and keeps all
+// its own state and its neighbours' (cons2prim, FP64) and keeps all
 // variables in registers; full face records are written once.  FP64 is held
 // to 256 VGPRs (2 waves per SIMD, no AGPR spills: 13.0 against 15.9 ms at
 // 48M cells); mixed precision runs at the compiler's choice (3 waves per
@@ -132,12 +138,13 @@ __device__ __forceinline__ int link_of(T w) {
 // come from the FP32 primitive records Wp written by the producer of U (the
 // same values, half the gathered bytes, no FP64 conversions here: 6.45
 // against 7.8 ms at 48M cells).
-template <int NEQ, typename T, bool VISC, bool HEAT, bool FROMW>
+template <int NEQ, typename T, bool VISC, bool HEAT, bool FROMW, int NS>
 __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
     pass_g(fk_params P, int64_t nc, const double *__restrict__ U, const float *__restrict__ Wp,
            const int32_t *__restrict__ link, const void *__restrict__ geo, const void *__restrict__ rvec,
            const double *__restrict__ vol, T *__restrict__ face,
-           T *__restrict__ gvel, double *__restrict__ dgrad, double *__restrict__ dtheta) {
+           T *__restrict__ gvel, double *__restrict__ dgrad, double *__restrict__ dtheta,
+           unsigned long long *__restrict__ health, int step) {
   const int64_t c = xcd_block() * kBlock + threadIdx.x;
   if (c >= nc) return;
   const fk::Mat<double> md = fk::make_mat<double>(P);
@@ -159,6 +166,16 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) W[v] = (T)Wd[v];
   }
+  {
+    // health of the state: phase (or mixture) densities positive and every
+    // primitive finite (a NaN fails the comparisons, an overflow makes the
+    // sum inf); the smallest (step, cell) that fails is kept
+    T sum = 0;
+#pragma unroll
+    for (int v = 0; v < NEQ; ++v) sum += W[v];
+    const bool ok = (NEQ == 10 ? (W[1] > 0 && W[2] > 0) : W[0] > 0) && fabs(sum) < (T)INFINITY;
+    if (!ok) atomicMin(health, ((unsigned long long)(unsigned)step << 32) | (unsigned long long)c);
+  }
   T g[NEQ][3], mn[NEQ], mx[NEQ];
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) {
@@ -167,32 +184,26 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
     mx[v] = (T)-1e12;
   }
   const T V = (T)vol[c];
-  // The 4 slot links are loaded before the face loop and rotated through
+  // The NS slot links are loaded before the face loop and rotated through
   // registers, so each iteration's neighbour gather does not wait on its link
   // load (rolled loop: 6.19 against 6.49 ms mixed, 12.8 against 12.9 ms FP64
   // at 48M cells).  Mixed: the link rides in rvec's 4th word (one 16-byte
   // load per slot); FP64 (32-byte rvec records) measured faster with the
   // 4-byte link array.
-  int lq0, lq1, lq2, lq3;
-  if constexpr (sizeof(T) == 4) {
-    lq0 = link_of<T>(ld4<T>(rvec, c * 4 + 0).w);
-    lq1 = link_of<T>(ld4<T>(rvec, c * 4 + 1).w);
-    lq2 = link_of<T>(ld4<T>(rvec, c * 4 + 2).w);
-    lq3 = link_of<T>(ld4<T>(rvec, c * 4 + 3).w);
-  } else {
-    lq0 = link[c * 4 + 0];
-    lq1 = link[c * 4 + 1];
-    lq2 = link[c * 4 + 2];
-    lq3 = link[c * 4 + 3];
+  int lq[NS];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) {
+    if constexpr (sizeof(T) == 4) lq[j] = link_of<T>(ld4<T>(rvec, c * NS + j).w);
+    else lq[j] = link[c * NS + j];
   }
 #pragma unroll 1
-  for (int j = 0; j < 4; ++j) {
-    const int64_t s = c * 4 + j;
-    const int lk = lq0;
-    lq0 = lq1;
-    lq1 = lq2;
-    lq2 = lq3;
-    const int nb = link_nb(lk);
+  for (int j = 0; j < NS; ++j) {
+    const int64_t s = c * NS + j;
+    const int lk = lq[0];
+#pragma unroll
+    for (int q = 0; q + 1 < NS; ++q) lq[q] = lq[q + 1];
+    if (NS > 4 && lk == kNoFace) break;  // past the cell's last face
+    const int nb = link_nb<NS>(lk);
     const bool own = link_own(lk);
     const V4<T> ga = ld4<T>(geo, s);
     const T n[3] = {ga.x, ga.y, ga.z};
@@ -246,9 +257,11 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
   T th[NEQ];
 #pragma unroll
   for (int v = 0; v < NEQ; ++v) th[v] = (T)1e9;
+  // slots past the last face have r = 0: phi = 1 there, which leaves theta
+  // (at most 1 after the first face) unchanged
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const V4<T> r = ld4<T>(rvec, c * 4 + j);
+  for (int j = 0; j < NS; ++j) {
+    const V4<T> r = ld4<T>(rvec, c * NS + j);
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) {
       const T wu = W[v] + r.x * g[v][0] + r.y * g[v][1] + r.z * g[v][2];
@@ -263,13 +276,13 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
   }
   // MUSCL face states
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    V4<T> r = ld4<T>(rvec, c * 4 + j);
+  for (int j = 0; j < NS; ++j) {
+    V4<T> r = ld4<T>(rvec, c * NS + j);
     // FP64: r g is also formed by the limiter loop; the empty asm keeps the
     // compiler from holding those 4 NEQ products in registers until here (it
     // recomputes them instead).  FP32 measured faster with them held.
     if constexpr (sizeof(T) == 8) asm volatile("" : "+v"(r.x), "+v"(r.y), "+v"(r.z));
-    T *out = face + (c * 4 + j) * NEQ;
+    T *out = face + (c * NS + j) * NEQ;
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) {
       const T ext = r.x * g[v][0] + r.y * g[v][1] + r.z * g[v][2];
@@ -285,7 +298,9 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
 #pragma unroll
       for (int d = 0; d < 3; ++d) G[i * 3 + d] = g[iu + i][d];
     if constexpr (HEAT) {
-      const T gm = (T)P.gamma[0], pi = (T)P.pInf[0];
+      const T gm = (T)P.gamma[0];
+      // FP32 path: W[4] is the deviation from pRef, absolute p = W[4] + pRef
+      const T pi = sizeof(T) == 4 ? (T)(P.pInf[0] + P.pRef) : (T)P.pInf[0];
       const T R = (T)P.Cp[0] * (gm - 1) / gm;
 #pragma unroll
       for (int d = 0; d < 3; ++d) G[9 + d] = (g[4][d] - ((W[4] + pi) / W[0]) * g[0][d]) / (W[0] * R);
@@ -325,7 +340,7 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
 // rlist (length in rcnt[5]); each wave stores its counts in rpart[wave]
 // ([n_chem, iterations], and Tsat iterations for 2); Wout gets the primitive
 // records of the cells relaxed here.
-template <int NEQ, typename T, bool VISC, bool HEAT, int MODE, bool STAGE, bool WOUT = false, int RLX = 0>
+template <int NEQ, typename T, bool VISC, bool HEAT, int MODE, bool STAGE, int NS, bool WOUT = false, int RLX = 0>
 __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3)) pass_f(fk_params P, int64_t nc, const double *__restrict__ U,
                                                  const double *__restrict__ U0, double *__restrict__ Unew,
                                                  const int32_t *__restrict__ link,
@@ -344,7 +359,7 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   const int64_t c = c0 + threadIdx.x;
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   T *sF = reinterpret_cast<T *>(lds_raw);
-  T *sG = sF + kBlock * 4 * NEQ;
+  T *sG = sF + kBlock * NS * NEQ;
   if constexpr (STAGE) {
     // the workgroup's face records and gradients are contiguous in HBM and in
     // LDS: copied by direct-to-LDS loads (global_load_lds: lane i of a wave
@@ -355,9 +370,9 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
     const int64_t ncell = (nc - c0) < kBlock ? (nc - c0) : kBlock;
     const int wbase = __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
     {
-      constexpr int kW = kBlock * 4 * NEQ * (int)sizeof(T) / 16;  // 16-byte words of a full block
-      const int nw = (int)(ncell * 4 * NEQ * (int64_t)sizeof(T) / 16);
-      const uint4 *src = reinterpret_cast<const uint4 *>(face + c0 * 4 * NEQ);
+      constexpr int kW = kBlock * NS * NEQ * (int)sizeof(T) / 16;  // 16-byte words of a full block
+      const int nw = (int)(ncell * NS * NEQ * (int64_t)sizeof(T) / 16);
+      const uint4 *src = reinterpret_cast<const uint4 *>(face + c0 * NS * NEQ);
       uint4 *dst = reinterpret_cast<uint4 *>(sF);
 #pragma unroll
       for (int k = 0; k < kW / kBlock; ++k) {
@@ -391,7 +406,7 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   }
   // face state of global slot `sl` of cell `cell`, from LDS when staged
   auto slot_ptr = [&](int64_t cell, int64_t sl) -> const T * {
-    if (STAGE && cell >= c0 && cell < c0 + kBlock) return sF + (sl - c0 * 4) * NEQ;
+    if (STAGE && cell >= c0 && cell < c0 + kBlock) return sF + (sl - c0 * NS) * NEQ;
     return face + sl * NEQ;
   };
   auto grad_ptr = [&](int64_t cell) -> const T * {
@@ -448,19 +463,22 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   if constexpr (VISC) {
     double t[NEQ];
     fk::cons2prim<NEQ>(md, Uc, t);
+    if constexpr (NEQ == 5 && sizeof(T) == 4) t[4] -= md.pr;  // FP32 path: pressure as deviation from pRef
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) Wc1[v] = (T)t[v];
   }
   // slot links loaded up front and rotated, as in pass G (mixed 6.85 against
   // 7.12 ms for the fused pass, FP64 14.4 against 14.8 ms)
-  int mq0 = link[c * 4 + 0], mq1 = link[c * 4 + 1], mq2 = link[c * 4 + 2], mq3 = link[c * 4 + 3];
-  for (int j = 0; j < 4; ++j) {
-    const int64_t s = c * 4 + j;
-    const int mv = mq0;
-    mq0 = mq1;
-    mq1 = mq2;
-    mq2 = mq3;
-    const int nb = link_nb(mv);
+  int mq[NS];
+#pragma unroll
+  for (int j = 0; j < NS; ++j) mq[j] = link[c * NS + j];
+  for (int j = 0; j < NS; ++j) {
+    const int64_t s = c * NS + j;
+    const int mv = mq[0];
+#pragma unroll
+    for (int q = 0; q + 1 < NS; ++q) mq[q] = mq[q + 1];
+    if (NS > 4 && mv == kNoFace) break;  // past the cell's last face
+    const int nb = link_nb<NS>(mv);
     const bool own = link_own(mv);
     const V4<T> ga = ld4<T>(geo, s);
     const T n[3] = {ga.x, ga.y, ga.z};
@@ -932,7 +950,7 @@ int require(const fk_ctx *c) {
 
 // upload the precision-dependent slot geometry
 int upload_geometry(fk_ctx *c) {
-  const int64_t ns = c->nc * 4;
+  const int64_t ns = c->nc * c->ns;
   if (c->p.precision == FK_MIXED) {
     std::vector<float> g((size_t)ns * 4), r((size_t)ns * 4), l((size_t)ns);
     for (size_t i = 0; i < g.size(); ++i) g[i] = (float)c->h_geo[i];
@@ -959,11 +977,12 @@ int upload_geometry(fk_ctx *c) {
 void free_all(fk_ctx *c) {
   void *ptrs[] = {c->link, c->geo, c->rvec, c->wls, c->vol, c->volcfl, c->U[0], c->U[1], c->U[2], c->face,
                   c->gvel, c->rhs, c->dtc, c->red, c->dscal, c->dgrad, c->dtheta, c->lcount, c->rlist,
-                  c->rpart, c->wprim};
+                  c->rpart, c->wprim, c->health};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->link = nullptr;
   c->wprim = nullptr;
+  c->health = nullptr;
   c->w_valid = false;
   c->rlist = nullptr;
   c->rpart = nullptr;
@@ -976,7 +995,7 @@ void free_all(fk_ctx *c) {
   c->lcount = nullptr;
 }
 
-template <typename T>
+template <typename T, int NS>
 int launch_pass_g_t(fk_ctx *c) {
   const unsigned gr = grid_xcd(c->nc);
   const bool visc = c->p.viscFlag == 1;
@@ -989,8 +1008,9 @@ int launch_pass_g_t(fk_ctx *c) {
   const bool fromw = sizeof(T) == 4 && c->prims_w && c->w_valid;
 #define PG2(NEQ, VISC, HEAT, FW)                                                                                     \
   LAUNCH_TIMED(c, "pass_g",                                                                                          \
-               hipLaunchKernelGGL((pass_g<NEQ, T, VISC, HEAT, FW>), dim3(gr), dim3(kBlock), 0, c->stream, c->p,      \
-                                  c->nc, U, c->wprim, c->link, c->geo, c->rvec, c->vol, face, gv, dg, dth))
+               hipLaunchKernelGGL((pass_g<NEQ, T, VISC, HEAT, FW, NS>), dim3(gr), dim3(kBlock), 0, c->stream, c->p,  \
+                                  c->nc, U, c->wprim, c->link, c->geo, c->rvec, c->vol, face, gv, dg, dth,  \
+                                  c->health, (int)std::min<int64_t>(c->step, INT32_MAX)))
 #define PG(NEQ, VISC, HEAT)                     \
   do {                                          \
     if constexpr (sizeof(T) == 4) {             \
@@ -1014,7 +1034,7 @@ int launch_pass_g_t(fk_ctx *c) {
   return FK_OK;
 }
 
-template <typename T, int MODE>
+template <typename T, int MODE, int NS>
 int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Unew, bool write_w = false,
                     int relax = 0) {
   const unsigned gr = grid_xcd(c->nc);
@@ -1029,9 +1049,10 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
   // fused relaxation (last stage): records of the relaxed state, mixed precision
   float *Wr = (sizeof(T) == 4 && c->prims_w) ? c->wprim : nullptr;
   unsigned long long *cnt = (unsigned long long *)c->lcount;
-  // LDS staging of face states (mixed precision only: 40 KB + 9 KB per workgroup)
-  const bool stage = c->stage_f && sizeof(T) == 4;
-  const size_t lds = stage ? (size_t)kBlock * (4 * c->neq + 12) * sizeof(T) : 0;
+  // LDS staging of face states (mixed precision, tetrahedral meshes: 40 KB +
+  // 9 KB per workgroup)
+  const bool stage = c->stage_f && sizeof(T) == 4 && NS == 4;
+  const size_t lds = stage ? (size_t)kBlock * (NS * c->neq + 12) * sizeof(T) : 0;
 #define PF_ARGS(W) \
   c->p, c->nc, U, U0, Unew, c->link, c->geo, wl, c->vol, c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp, W
 #define PF(NEQ, VISC, HEAT)                                                                                            \
@@ -1040,22 +1061,22 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
       if (relax == 1) {                                                                                                \
         if (stage)                                                                                                     \
           LAUNCH_TIMED(c, "pass_f_relax",                                                                              \
-                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, true, false, 1>), dim3(gr), dim3(kBlock),   \
+                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, (NS == 4), NS, false, 1>), dim3(gr), dim3(kBlock),   \
                                           lds, c->stream, PF_ARGS(Wr), c->rlist, cnt, c->rpart));                      \
         else                                                                                                           \
           LAUNCH_TIMED(c, "pass_f_relax",                                                                              \
-                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false, false, 1>), dim3(gr), dim3(kBlock),  \
+                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false, NS, false, 1>), dim3(gr), dim3(kBlock),  \
                                           0, c->stream, PF_ARGS(Wr), c->rlist, cnt, c->rpart));                        \
         break;                                                                                                         \
       }                                                                                                                \
       if (relax == 2) {                                                                                                \
         if (stage)                                                                                                     \
           LAUNCH_TIMED(c, "pass_f_relax",                                                                              \
-                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, true, false, 2>), dim3(gr), dim3(kBlock),   \
+                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, (NS == 4), NS, false, 2>), dim3(gr), dim3(kBlock),   \
                                           lds, c->stream, PF_ARGS(Wr), c->rlist, cnt, c->rpart));                      \
         else                                                                                                           \
           LAUNCH_TIMED(c, "pass_f_relax",                                                                              \
-                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false, false, 2>), dim3(gr), dim3(kBlock),  \
+                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false, NS, false, 2>), dim3(gr), dim3(kBlock),  \
                                           0, c->stream, PF_ARGS(Wr), c->rlist, cnt, c->rpart));                        \
         break;                                                                                                         \
       }                                                                                                                \
@@ -1063,21 +1084,21 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
     if constexpr (MODE == 0 && sizeof(T) == 4) {                                                                       \
       if (Wout) {                                                                                                      \
         if (stage)                                                                                                     \
-          LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, true, true>), dim3(gr),             \
+          LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, (NS == 4), NS, true>), dim3(gr),             \
                                                  dim3(kBlock), lds, c->stream, PF_ARGS(Wout), nullptr, nullptr,         \
                                                  nullptr));                                                             \
         else                                                                                                           \
-          LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false, true>), dim3(gr),            \
+          LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false, NS, true>), dim3(gr),            \
                                                  dim3(kBlock), 0, c->stream, PF_ARGS(Wout), nullptr, nullptr,           \
                                                  nullptr));                                                             \
         break;                                                                                                         \
       }                                                                                                                \
     }                                                                                                                  \
     if (stage)                                                                                                         \
-      LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, true>), dim3(gr), dim3(kBlock), lds,     \
+      LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, (NS == 4), NS>), dim3(gr), dim3(kBlock), lds,     \
                                              c->stream, PF_ARGS(Wout), nullptr, nullptr, nullptr));                     \
     else                                                                                                               \
-      LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false>), dim3(gr), dim3(kBlock), 0,      \
+      LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false, NS>), dim3(gr), dim3(kBlock), 0,      \
                                              c->stream, PF_ARGS(Wout), nullptr, nullptr, nullptr));                     \
   } while (0)
   if (c->neq == 5) {
@@ -1093,13 +1114,25 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
   return FK_OK;
 }
 
+// the kernels for the mesh's slots per cell (4 or 6)
+template <typename T>
+int launch_pass_g_ns(fk_ctx *c) {
+  return c->ns == 4 ? launch_pass_g_t<T, 4>(c) : launch_pass_g_t<T, 6>(c);
+}
+template <typename T, int MODE>
+int launch_pass_f_ns(fk_ctx *c, int irk, double dt, const double *dtp, double *Unew, bool write_w = false,
+                     int relax = 0) {
+  return c->ns == 4 ? launch_pass_f_t<T, MODE, 4>(c, irk, dt, dtp, Unew, write_w, relax)
+                    : launch_pass_f_t<T, MODE, 6>(c, irk, dt, dtp, Unew, write_w, relax);
+}
+
 int do_pass_g(fk_ctx *c) {
-  return c->p.precision == FK_MIXED ? launch_pass_g_t<float>(c) : launch_pass_g_t<double>(c);
+  return c->p.precision == FK_MIXED ? launch_pass_g_ns<float>(c) : launch_pass_g_ns<double>(c);
 }
 
 int do_pass_f_rhs(fk_ctx *c) {
-  int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 1>(c, 0, 0.0, nullptr, nullptr)
-                                      : launch_pass_f_t<double, 1>(c, 0, 0.0, nullptr, nullptr);
+  int rc = c->p.precision == FK_MIXED ? launch_pass_f_ns<float, 1>(c, 0, 0.0, nullptr, nullptr)
+                                      : launch_pass_f_ns<double, 1>(c, 0, 0.0, nullptr, nullptr);
   if (rc) return rc;
   c->rhs_f32 = c->p.precision == FK_MIXED;
   c->rhs_valid = true;
@@ -1116,8 +1149,8 @@ int do_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want_w);
 int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want_w, int relax = 0) {
   if (relax) {
     const int nb = next_buffer(c);
-    int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 0>(c, irk, dt, dtp, c->U[nb], false, relax)
-                                        : launch_pass_f_t<double, 0>(c, irk, dt, dtp, c->U[nb], false, relax);
+    int rc = c->p.precision == FK_MIXED ? launch_pass_f_ns<float, 0>(c, irk, dt, dtp, c->U[nb], false, relax)
+                                        : launch_pass_f_ns<double, 0>(c, irk, dt, dtp, c->U[nb], false, relax);
     if (rc) return rc;
     c->ia = nb;
     c->w_valid = false;  // until stage 2 has written the listed cells' records
@@ -1125,8 +1158,8 @@ int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want
     return FK_OK;
   }
   if (!c->fuse_update) {
-    int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 2>(c, 0, 0.0, nullptr, nullptr)
-                                        : launch_pass_f_t<double, 2>(c, 0, 0.0, nullptr, nullptr);
+    int rc = c->p.precision == FK_MIXED ? launch_pass_f_ns<float, 2>(c, 0, 0.0, nullptr, nullptr)
+                                        : launch_pass_f_ns<double, 2>(c, 0, 0.0, nullptr, nullptr);
     if (rc) return rc;
     c->rhs_f32 = false;
     rc = do_update(c, irk, dt, dtp, want_w);
@@ -1135,8 +1168,8 @@ int do_pass_f_update(fk_ctx *c, int irk, double dt, const double *dtp, bool want
   }
   const int nb = next_buffer(c);
   const bool w = want_w && c->p.precision == FK_MIXED && c->prims_w;
-  int rc = c->p.precision == FK_MIXED ? launch_pass_f_t<float, 0>(c, irk, dt, dtp, c->U[nb], w)
-                                      : launch_pass_f_t<double, 0>(c, irk, dt, dtp, c->U[nb], false);
+  int rc = c->p.precision == FK_MIXED ? launch_pass_f_ns<float, 0>(c, irk, dt, dtp, c->U[nb], w)
+                                      : launch_pass_f_ns<double, 0>(c, irk, dt, dtp, c->U[nb], false);
   if (rc) return rc;
   c->ia = nb;
   c->w_valid = w;
@@ -1328,6 +1361,7 @@ int check_params(const fk_params *p) {
   const bool inlet_default = p->inletP == 0 && p->inletT == 0;
   if (!inlet_default && !(p->inletT > 0 && std::isfinite(p->inletP) && std::isfinite(p->inletT)))
     return fail(FK_EINVAL, "inletT must be > 0 and inletP finite (or both 0 for the default inlet)");
+  if (!std::isfinite(p->pRef) || std::fabs(p->pRef) > 3e38) return fail(FK_EINVAL, "pRef must be finite (FP32 range)");
   return FK_OK;
 }
 
@@ -1335,6 +1369,8 @@ int check_params(const fk_params *p) {
 // reference's constants (bc.h:5-7 of either model) when both fields are 0.
 fk_params resolved(const fk_params &p) {
   fk_params r = p;
+  // one value for the FP64 conversions and the FP32 flux code
+  r.pRef = (double)(float)p.pRef;
   if (r.inletP == 0 && r.inletT == 0) {
     const bool sp = r.model == FK_MODEL_SINGLE;
     r.inletP = sp ? 4.2 * 101300 : 11.4e5;
@@ -1415,11 +1451,10 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
   if (!c || !fc2el || !boundCond || !n || !area || !fcCenter || !cellCenter || !volume)
     return fail(FK_EINVAL, "null argument");
   if (nElem <= 0 || nFcs <= 0) return fail(FK_EINVAL, "empty mesh");
-  if (nElem * 4 >= (int64_t)1 << 30) return fail(FK_ERANGE, "mesh too large for 30-bit slot ids");
+  if (nElem >= INT32_MAX) return fail(FK_ERANGE, "mesh too large for int32 cell ids");
   HIPCHK(hipSetDevice(c->device));
   HIPCHK(hipStreamSynchronize(c->stream));
-  // cell -> faces in ascending face id
-  std::vector<int32_t> sf((size_t)(nElem * 4), -1);
+  // faces per cell: 4 (tetrahedron), 5 (pyramid, prism) or 6 (hexahedron)
   std::vector<uint8_t> cnt((size_t)nElem, 0);
   for (int64_t f = 0; f < nFcs; ++f) {
     const int32_t o = fc2el[2 * f], nb = fc2el[2 * f + 1];
@@ -1430,28 +1465,45 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
     for (int s = 0; s < 2; ++s) {
       const int32_t cc = fc2el[2 * f + s];
       if (cc < 0) continue;
-      if (cnt[(size_t)cc] >= 4) return fail(FK_EINVAL, "cell with more than four faces (only tetrahedra)");
-      sf[(size_t)(cc * 4 + cnt[(size_t)cc]++)] = (int32_t)f;
+      if (cnt[(size_t)cc] >= 6) return fail(FK_EINVAL, "cell with more than six faces");
+      cnt[(size_t)cc]++;
     }
   }
-  for (int64_t cc = 0; cc < nElem; ++cc)
-    if (cnt[(size_t)cc] != 4) return fail(FK_EINVAL, "every cell must have exactly four faces (tetrahedra)");
-  const int64_t ns = nElem * 4;
-  std::vector<int32_t> hlink((size_t)ns);
+  int NS = 4;
+  for (int64_t cc = 0; cc < nElem; ++cc) {
+    if (cnt[(size_t)cc] < 4) return fail(FK_EINVAL, "every cell needs four to six faces");
+    if (cnt[(size_t)cc] != 4) NS = 6;
+  }
+  if (nElem * NS >= (int64_t)1 << 30) return fail(FK_ERANGE, "mesh too large for 30-bit slot ids");
+  // cell -> faces in ascending face id, slots past a cell's last face empty
+  std::vector<int32_t> sf((size_t)(nElem * NS), -1);
+  std::fill(cnt.begin(), cnt.end(), 0);
+  for (int64_t f = 0; f < nFcs; ++f)
+    for (int s = 0; s < 2; ++s) {
+      const int32_t cc = fc2el[2 * f + s];
+      if (cc >= 0) sf[(size_t)(cc * NS + cnt[(size_t)cc]++)] = (int32_t)f;
+    }
+  const int64_t ns = nElem * NS;
+  std::vector<int32_t> hlink((size_t)ns, kNoFace);
   c->h_geo.assign((size_t)ns * 4, 0.0);
   c->h_r.assign((size_t)ns * 4, 0.0);
   c->h_ls.assign((size_t)ns, 0.0);
   // face -> slot of the owner side / neighbour side
   std::vector<int32_t> fslot((size_t)(nFcs * 2), -1);
   for (int64_t s = 0; s < ns; ++s) {
-    const int64_t cc = s / 4;
     const int32_t f = sf[(size_t)s];
+    if (f < 0) continue;
+    const int64_t cc = s / NS;
     const int side = fc2el[2 * f] == cc ? 0 : 1;
     fslot[(size_t)(2 * f + side)] = (int32_t)s;
   }
   for (int64_t s = 0; s < ns; ++s) {
-    const int64_t cc = s / 4;
     const int32_t f = sf[(size_t)s];
+    if (f < 0) {  // no face: zero geometry, the empty-slot link
+      c->h_r[(size_t)(s * 4 + 3)] = (double)kNoFace;
+      continue;
+    }
+    const int64_t cc = s / NS;
     const int side = fc2el[2 * f] == cc ? 0 : 1;
     const int32_t other = fc2el[2 * f + (1 - side)];
     if (other >= 0) {
@@ -1478,6 +1530,7 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
   c->have_mesh = c->have_state = false;
   c->nc = nElem;
   c->nf = nFcs;
+  c->ns = NS;
   const int neq = c->neq;
   const size_t cs = (size_t)nElem;
   HIPCHK(hipMalloc(&c->link, (size_t)ns * 4));
@@ -1494,6 +1547,11 @@ int fk_upload_mesh(fk_ctx *c, int64_t nElem, int64_t nFcs, const int32_t *fc2el,
   HIPCHK(hipMalloc(&c->red, 4096 * 8));
   HIPCHK(hipMalloc(&c->dscal, 16 * 8));
   HIPCHK(hipMalloc(&c->lcount, 16 * 8));
+  HIPCHK(hipMalloc(&c->health, sizeof(unsigned long long)));
+  {
+    const unsigned long long none = ~0ull;
+    HIPCHK(hipMemcpy(c->health, &none, sizeof none, hipMemcpyHostToDevice));
+  }
   HIPCHK(hipMalloc(&c->wprim, cs * neq * sizeof(float)));
   HIPCHK(hipMemset(c->lcount, 0, 16 * 8));
   HIPCHK(hipMemcpy(c->link, hlink.data(), (size_t)ns * 4, hipMemcpyHostToDevice));
@@ -1693,9 +1751,12 @@ int fk_run_steps(fk_ctx *c, int nsteps, double *dt_last, double *phys_time) {
   // stage runs, so a failure leaves the state and the clock untouched
   if (p.model == FK_MODEL_TWO_PHASE && p.relaxFlag == 1 && nsteps > 0 && (rc = relax_preflight(c, p.relaxMode)))
     return rc;
-  // the device scalars hold dt and the physical time
+  // the device scalars hold dt and the physical time; the health record
+  // starts clean for this call
   double h[2] = {c->dt, c->time};
   HIPCHK(hipMemcpy(c->dscal, h, sizeof h, hipMemcpyHostToDevice));
+  unsigned long long hh = ~0ull;
+  HIPCHK(hipMemcpy(c->health, &hh, sizeof hh, hipMemcpyHostToDevice));
   for (int s = 0; s < nsteps; ++s) {
     if (p.rkSteps > 1) c->iu0 = c->ia;
     const bool newdt = p.adaptDt == 1 && c->step % p.adaptDtItr == 0;
@@ -1739,6 +1800,7 @@ int fk_run_steps(fk_ctx *c, int nsteps, double *dt_last, double *phys_time) {
     c->step++;
   }
   HIPCHK(hipMemcpyAsync(h, c->dscal, sizeof h, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipMemcpyAsync(&hh, c->health, sizeof hh, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(hipStreamSynchronize(c->stream));
   if (p.adaptDt == 1) {
     c->dt = h[0];
@@ -1748,6 +1810,25 @@ int fk_run_steps(fk_ctx *c, int nsteps, double *dt_last, double *phys_time) {
   }
   if (dt_last) *dt_last = c->dt;
   if (phys_time) *phys_time = c->time;
+  if (hh != ~0ull)
+    return fail(FK_ERANGE, "non-physical state: non-positive density or non-finite value from step " +
+                               std::to_string(hh >> 32) + " on (cell " + std::to_string(hh & 0xffffffffull) + ")");
+  return FK_OK;
+}
+
+int fk_health(fk_ctx *c, int64_t *step, int64_t *cell, int reset) {
+  int rc = require(c);
+  if (rc) return rc;
+  HIPCHK(hipSetDevice(c->device));
+  unsigned long long hh = 0;
+  HIPCHK(hipMemcpyAsync(&hh, c->health, sizeof hh, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(hipStreamSynchronize(c->stream));
+  if (step) *step = hh == ~0ull ? -1 : (int64_t)(hh >> 32);
+  if (cell) *cell = hh == ~0ull ? -1 : (int64_t)(hh & 0xffffffffull);
+  if (reset) {
+    const unsigned long long none = ~0ull;
+    HIPCHK(hipMemcpy(c->health, &none, sizeof none, hipMemcpyHostToDevice));
+  }
   return FK_OK;
 }
 
@@ -1804,7 +1885,7 @@ int fk_download(fk_ctx *c, const char *name, void *host, int64_t bytes) {
   if ((s == "grad" || s == "theta") && !c->dgrad) return fail(FK_ESTATE, "enable fk_set_debug before pass G");
   if (s == "grad") return get(c->dgrad, nc * neq * 3 * 8);
   if (s == "theta") return get(c->dtheta, nc * neq * 8);
-  if (s == "face") return get(c->face, nc * 4 * neq * (int64_t)ts);
+  if (s == "face") return get(c->face, nc * c->ns * neq * (int64_t)ts);
   if (s == "rhs") {
     if (!c->rhs_f32) return get(c->rhs, nc * neq * 8);
     // stored as FP32 by the mixed dt stage: widen on the host
@@ -1826,7 +1907,7 @@ int fk_download(fk_ctx *c, const char *name, void *host, int64_t bytes) {
     return FK_OK;
   }
   if (s == "slots") {
-    if (bytes != nc * 4 * 4) return fail(FK_EINVAL, "size mismatch for slots");
+    if (bytes != nc * c->ns * 4) return fail(FK_EINVAL, "size mismatch for slots");
     std::memcpy(host, c->slot_face.data(), (size_t)bytes);
     return FK_OK;
   }

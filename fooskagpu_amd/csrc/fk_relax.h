@@ -409,10 +409,19 @@ FK_HD inline void therm_cold(const fk_params &P, const double *U, const ThermPro
   xe = t.enr[t.k];
 }
 
+// A phase state inside the stiffened gas's domain: T > 0 and p + pinf > 0
+// (a real sound speed).  The equilibrium equations also have roots outside
+// it; the surrogate's solves reject those and leave the cell to the
+// reference solve.
+FK_HD inline bool eos_domain(const fk_params &P, int k, double rho, double e) {
+  return eos_T(P, k, rho, e) > 0 && eos_p(P, k, rho, e) + P.pInf[k] > 0;
+}
+
 // Newton iterations with the analytic Jacobian on (rho_k, e_k) from (xr, xe)
 // until |d rho_k| <= tol |rho_k| and |d e_k| <= tol |e_k|.  Returns the
 // iterations (> 0), or -1 (no convergence within itmax, singular Jacobian,
-// or an iterate outside rho_k > ar_k); xr, xe hold the last iterate.
+// an iterate outside rho_k > ar_k, or a root outside either phase's EOS
+// domain); xr, xe hold the last iterate.
 // Residuals p_k - p_m and T_k - T_m (signs as thermRlx.cu:95-103) with
 //   p = (g - 1) rho (e - eta) - g pinf,  T = (e - eta - pinf / rho) / Cv,
 // the stiffened-gas EOS of eqos.h rearranged so that an iteration takes
@@ -454,7 +463,10 @@ FK_HD inline int therm_newton(const fk_params &P, const ThermProb &t, double &xr
     xr -= dr;
     xe -= dee;
     if (!(xr > t.ar[k]) || !__builtin_isfinite(xe)) return -1;
-    if (fabs(dr) <= tol * fabs(xr) && fabs(dee) <= tol * fabs(xe)) return it;
+    if (fabs(dr) <= tol * fabs(xr) && fabs(dee) <= tol * fabs(xe)) {
+      const double rmf = t.ar[pm] / (1 - ark / xr), emf = em0 - ratio * (xe - ek0);
+      return (eos_domain(P, k, xr, xe) && eos_domain(P, pm, rmf, emf)) ? it : -1;
+    }
   }
   return -1;
 }
@@ -601,6 +613,7 @@ FK_HD inline int chem_polish(const fk_params &P, double *U, double ak0, double r
       x[pm + 2] = (ch.rhoMix - s2) / x[pm];
       x[pm + 4] = (ch.rEint - s3) / (x[pm] * x[pm + 2]);
       if (!(x[pm + 2] > 0)) return -1;
+      if (!eos_domain(P, 0, x[2], x[4]) || !eos_domain(P, 1, x[3], x[5])) return -1;
       U[0] = clampd(P, x[0]);
       U[1] = x[0] * x[2];
       U[2] = x[1] * x[3];

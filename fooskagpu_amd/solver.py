@@ -49,6 +49,10 @@ class GpuSolver:
     def upload_mesh(self, m):
         self.mesh = m
         self.n_cells = m.n_cells
+        # face slots per cell on the device: 4 on meshes of tetrahedra, else 6
+        faces = np.bincount(m.fc2el[:, 0], minlength=m.n_cells) + np.bincount(
+            m.fc2el[m.fc2el[:, 1] >= 0, 1], minlength=m.n_cells)
+        self.n_slots = 4 if np.all(faces == 4) else 6
         shift = np.ascontiguousarray(m.shift) if m.periodic else None
         self._keep = [np.ascontiguousarray(a) for a in (m.fc2el, m.bc, m.n, m.area, m.fc_center, m.cell_center,
                                                         m.volume)]
@@ -108,6 +112,13 @@ class GpuSolver:
         _lib.check_hip(_lib.hip_lib().fk_cons_calc(self._h, _lib.dptr(out)))
         return out
 
+    def health(self, reset=False):
+        """(first step, lowest cell) of a non-physical state found by pass G
+        since the last reset, or (-1, -1)."""
+        st, ce = ctypes.c_int64(), ctypes.c_int64()
+        _lib.check_hip(_lib.hip_lib().fk_health(self._h, ctypes.byref(st), ctypes.byref(ce), 1 if reset else 0))
+        return st.value, ce.value
+
     def run_steps(self, nsteps):
         """explicitLoop iterations on the device; returns (last dt, time)."""
         dt, t = ctypes.c_double(), ctypes.c_double()
@@ -134,8 +145,8 @@ class GpuSolver:
         C, E = self.n_cells, self.neq
         f32 = self._p.precision == _lib.MIXED
         shapes = {"grad": ((C, E, 3), np.float64), "theta": ((C, E), np.float64),
-                  "face": ((C, 4, E), np.float32 if f32 else np.float64), "rhs": ((C, E), np.float64),
-                  "signal": ((C,), np.float64), "slots": ((C, 4), np.int32)}
+                  "face": ((C, self.n_slots, E), np.float32 if f32 else np.float64), "rhs": ((C, E), np.float64),
+                  "signal": ((C,), np.float64), "slots": ((C, self.n_slots), np.int32)}
         shape, dt = shapes[name]
         out = np.empty(shape, dtype=dt)
         _lib.check_hip(_lib.hip_lib().fk_download(self._h, name.encode(), out.ctypes.data_as(ctypes.c_void_p),

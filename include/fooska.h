@@ -77,6 +77,14 @@ typedef struct fk_params {
    * the reference's constants, 11.4e5 Pa / 115.3 K (two-phase) and
    * 4.2 * 101300 Pa / 288 K (single-phase). */
   double inletP, inletT;
+  /* Reference pressure of the single-phase mixed-precision path (no
+   * reference counterpart): its FP32 face pressures hold p - pRef, so that a
+   * flow with small pressure differences on a large background (Mach 0.1:
+   * p ~ 71, differences ~ 0.06) keeps FP32 relative precision in them; the
+   * absolute pressure is formed for sound speeds and the energy flux.  Set it
+   * near the mean pressure; it is rounded to the nearest FP32 value.  0 =
+   * none.  Ignored by the FP64 path and the two-phase model. */
+  double pRef;
 } fk_params;
 
 typedef struct fk_ctx fk_ctx;
@@ -102,8 +110,10 @@ int fk_set_params(fk_ctx *ctx, const fk_params *p);
  * fc2el[nFcs][2] (owner < neighbour, -1 on boundary faces), boundCond[nFcs],
  * n[nFcs][3] unit normal out of the owner, area[nFcs], fcCenter[nFcs][3],
  * cellCenter[nElem][3], volume[nElem]; shift[nFcs][3] (periodic translation
- * of the face as seen from the neighbour) may be NULL.  Only tetrahedra:
- * every cell must have exactly four faces. */
+ * of the face as seen from the neighbour) may be NULL.  Cells have four to
+ * six faces (tetrahedra, pyramids, prisms, hexahedra, translator.cu:122-293);
+ * every per-cell sum runs over the cell's faces in ascending face id.  A
+ * mesh of tetrahedra only keeps 4 face slots per cell, any other mesh 6. */
 int fk_upload_mesh(fk_ctx *ctx, int64_t nElem, int64_t nFcs, const int32_t *fc2el, const int32_t *boundCond,
                    const double *n, const double *area, const double *fcCenter, const double *cellCenter,
                    const double *volume, const double *shift);
@@ -156,8 +166,18 @@ int fk_cons_calc_noalloc(fk_ctx *ctx, double *out);
 /* Whole time steps in explicitLoop order (dt is taken before the stage-0
  * update of every adaptDtItr-th step), with the relaxation of relaxMode
  * after the stages.  Returns the last dt and the time.  FK_RELAX_MLP
- * without weights fails with FK_ESTATE before any stage runs. */
+ * without weights fails with FK_ESTATE before any stage runs.  Every pass G
+ * checks the state it reads (phase densities, or the density, positive and
+ * every primitive finite); if a check failed during the call, fk_run_steps
+ * still completes the steps and advances the clock but returns FK_ERANGE,
+ * naming the first such step and its lowest cell (fk_health).  The reference has no
+ * such check (it runs on, explicitLoop.cu:66-69 reports only launch errors). */
 int fk_run_steps(fk_ctx *ctx, int nsteps, double *dt_last, double *phys_time);
+/* Health record of pass G since the last reset (fk_run_steps resets it when
+ * it starts): the first step with a non-physical state and the lowest such
+ * cell in that step, or -1 for both when every check passed; reset != 0
+ * clears it. */
+int fk_health(fk_ctx *ctx, int64_t *step, int64_t *cell, int reset);
 
 /* Block until the context's stream is idle. */
 int fk_sync(fk_ctx *ctx);
@@ -173,10 +193,12 @@ int fk_set_clock(fk_ctx *ctx, double dt, double time, int64_t step);
  * layouts (FP64 path; the mixed path stores FP32 where marked f):
  *  "grad"   [nElem][NEQ][3]   Green-Gauss gradients (needs fk_set_debug(1))
  *  "theta"  [nElem][NEQ]      limiter (needs fk_set_debug(1))
- *  "face"   [nElem][4][NEQ]   reconstructed face states per cell face slot (f)
+ *  "face"   [nElem][NS][NEQ]  reconstructed face states per cell face slot (f);
+ *                             NS = 4 on tetrahedral meshes, else 6 (slots past
+ *                             a cell's last face hold no face state)
  *  "rhs"    [nElem][NEQ]      residual of the last fk_compute_rhs (f); mixed
  *                             precision stores it as FP32 (widened here)
- *  "slots"  [nElem][4] int32  face id of each cell face slot
+ *  "slots"  [nElem][NS] int32 face id of each cell face slot, -1 past the last
  *  "signal" [nElem]           max signal speed per cell                      */
 int fk_download(fk_ctx *ctx, const char *name, void *host, int64_t bytes);
 int fk_set_debug(fk_ctx *ctx, int on);

@@ -14,6 +14,9 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <new>
 #include <string>
 #include <vector>
@@ -830,7 +833,9 @@ struct orc {
   int64_t nc = 0, nf = 0;
   std::vector<int32_t> fc2el, bc;
   std::vector<double> n, area, fcc, ccc, vol, shift, volCfl, waleLs;
-  std::vector<int32_t> cf;  // [nc][4] faces of each cell, ascending
+  int ns = 4;                // face slots per cell: 4 (all tetrahedra) or 6
+  std::vector<int32_t> cf;   // [nc][ns] faces of each cell, ascending, -1 past the last
+  std::vector<int8_t> ncf;   // [nc] faces of each cell (4 tet, 5 pyramid / prism, 6 hex)
   std::vector<double> U, U0, primF, grad, wmin, wmax, theta, ext, rhs, signal;
   std::vector<FaceFlux> ff;
   std::vector<double> visc;  // [nf][5]: momentum x,y,z, energy, heat
@@ -878,21 +883,32 @@ int orc_create(const fk_params *p, int64_t nElem, int64_t nFcs, const int32_t *f
     if (shift) o->shift.assign(shift, shift + 3 * nFcs);
     else o->shift.assign((size_t)(3 * nFcs), 0.0);
     std::vector<int> cnt((size_t)nElem, 0);
-    o->cf.assign((size_t)(4 * nElem), -1);
+    for (int64_t f = 0; f < nFcs; ++f)
+      for (int s = 0; s < 2; ++s) {
+        int32_t c = fc2el[2 * f + s];
+        if (c < 0) continue;
+        if (c >= nElem || cnt[(size_t)c] >= 6) {
+          delete o;
+          return fail(FK_EINVAL, "cell with more than six faces or bad fc2el");
+        }
+        cnt[(size_t)c]++;
+      }
+    o->ns = 4;
+    for (int64_t c = 0; c < nElem; ++c) {
+      if (cnt[(size_t)c] < 4) {
+        delete o;
+        return fail(FK_EINVAL, "every cell needs four to six faces");
+      }
+      if (cnt[(size_t)c] != 4) o->ns = 6;
+    }
+    const int ns = o->ns;
+    o->cf.assign((size_t)(ns * nElem), -1);
+    o->ncf.assign((size_t)nElem, 0);
     for (int64_t f = 0; f < nFcs; ++f)  // ascending face id per cell
       for (int s = 0; s < 2; ++s) {
         int32_t c = fc2el[2 * f + s];
         if (c < 0) continue;
-        if (c >= nElem || cnt[(size_t)c] >= 4) {
-          delete o;
-          return fail(FK_EINVAL, "cell with more than four faces or bad fc2el");
-        }
-        o->cf[(size_t)(4 * c + cnt[(size_t)c]++)] = (int32_t)f;
-      }
-    for (int64_t c = 0; c < nElem; ++c)
-      if (cnt[(size_t)c] != 4) {
-        delete o;
-        return fail(FK_EINVAL, "every cell must have four faces");
+        o->cf[(size_t)(ns * c + o->ncf[(size_t)c]++)] = (int32_t)f;
       }
     o->volCfl.resize((size_t)nElem);
     for (int64_t c = 0; c < nElem; ++c) o->volCfl[(size_t)c] = std::pow(volume[c], 0.3333);
@@ -975,8 +991,8 @@ int orc_compute_face_values(orc *o) {
       mx[v] = -1e12;
       g[3 * v] = g[3 * v + 1] = g[3 * v + 2] = 0.0;
     }
-    for (int k = 0; k < 4; ++k) {
-      int64_t f = o->cf[(size_t)(4 * c + k)];
+    for (int k = 0; k < o->ncf[(size_t)c]; ++k) {
+      int64_t f = o->cf[(size_t)(o->ns * c + k)];
       int side = o->fc2el[(size_t)(2 * f)] == c ? 0 : 1;
       const double *P0 = &o->primF[(size_t)(f * 2 * neq)], *P1 = P0 + neq;
       const double *nv = &o->n[(size_t)(3 * f)];
@@ -998,8 +1014,8 @@ int orc_compute_face_values(orc *o) {
   for (int64_t c = 0; c < nc; ++c) {
     double *th = &o->theta[(size_t)(c * neq)];
     for (int v = 0; v < neq; ++v) th[v] = 1e9;
-    for (int k = 0; k < 4; ++k) {
-      int64_t f = o->cf[(size_t)(4 * c + k)];
+    for (int k = 0; k < o->ncf[(size_t)c]; ++k) {
+      int64_t f = o->cf[(size_t)(o->ns * c + k)];
       int id = o->fc2el[(size_t)(2 * f)] == c ? 0 : 1;
       const double *sh = &o->shift[(size_t)(3 * f)];
       double xf = o->fcc[(size_t)(3 * f)], yf = o->fcc[(size_t)(3 * f + 1)], zf = o->fcc[(size_t)(3 * f + 2)];
@@ -1134,8 +1150,8 @@ int orc_compute_rhs(orc *o) {
     for (int v = 0; v < neq; ++v) r[v] = 0.0;
     double sig = -1e9;
     const double *Uc = &o->U[(size_t)(c * neq)];
-    for (int k = 0; k < 4; ++k) {
-      int64_t f = o->cf[(size_t)(4 * c + k)];
+    for (int k = 0; k < o->ncf[(size_t)c]; ++k) {
+      int64_t f = o->cf[(size_t)(o->ns * c + k)];
       int side = o->fc2el[(size_t)(2 * f)] == c ? 0 : 1;
       const double *nv = &o->n[(size_t)(3 * f)];
       double A = o->area[(size_t)f];
@@ -1174,8 +1190,8 @@ int orc_compute_rhs(orc *o) {
     // no fixed order)
     if (visc) {
       double rv[5] = {0, 0, 0, 0, 0};  // momentum x,y,z, energy, heat
-      for (int k = 0; k < 4; ++k) {
-        int64_t f = o->cf[(size_t)(4 * c + k)];
+      for (int k = 0; k < o->ncf[(size_t)c]; ++k) {
+        int64_t f = o->cf[(size_t)(o->ns * c + k)];
         int side = o->fc2el[(size_t)(2 * f)] == c ? 0 : 1;
         double A = o->area[(size_t)f];
         const double *vv = &o->visc[(size_t)(f * 5)];
@@ -1303,14 +1319,20 @@ int orc_get(const orc *o, const char *name, void *out, int64_t bytes) {
   if (s == "rhs") return put(o->rhs.data(), (int64_t)(o->rhs.size() * 8));
   if (s == "signal") return put(o->signal.data(), (int64_t)(o->signal.size() * 8));
   if (s == "slots") return put(o->cf.data(), (int64_t)(o->cf.size() * 4));
+  if (s == "nslots") return put(&o->ns, 4);
   if (s == "face") {
-    if (bytes != o->nc * 4 * neq * 8) return fail(FK_EINVAL, "size mismatch");
+    const int ns = o->ns;
+    if (bytes != o->nc * ns * neq * 8) return fail(FK_EINVAL, "size mismatch");
     double *d = (double *)out;
     for (int64_t c = 0; c < o->nc; ++c)
-      for (int k = 0; k < 4; ++k) {
-        int64_t f = o->cf[(size_t)(4 * c + k)];
+      for (int k = 0; k < ns; ++k) {
+        int64_t f = o->cf[(size_t)(ns * c + k)];
+        if (f < 0) {  // no face in this slot
+          for (int v = 0; v < neq; ++v) d[(c * ns + k) * neq + v] = 0.0;
+          continue;
+        }
         int side = o->fc2el[(size_t)(2 * f)] == c ? 0 : 1;
-        for (int v = 0; v < neq; ++v) d[(c * 4 + k) * neq + v] = o->primF[(size_t)(f * 2 * neq + side * neq + v)];
+        for (int v = 0; v < neq; ++v) d[(c * ns + k) * neq + v] = o->primF[(size_t)(f * 2 * neq + side * neq + v)];
       }
     return FK_OK;
   }
@@ -1364,6 +1386,14 @@ int orc_relax_cell(const fk_params *p, double *U, int *iters, int *tsat_iters) {
   if (iters) *iters = it;
   if (tsat_iters) *tsat_iters = ts;
   return k;
+}
+
+int orc_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
 }
 
 }  // extern "C"

@@ -49,9 +49,10 @@ int orc_cons_calc(const orc *o, double *out);
 int orc_run_steps(orc *o, int nsteps, double *dt_last, double *phys_time);
 
 /* Intermediates: "grad" [C][NEQ][3], "theta" [C][NEQ], "wmin"/"wmax" [C][NEQ],
- * "primF" [F][2][NEQ], "face" [C][4][NEQ] (reconstructed state of each cell
- * face slot, faces in ascending id), "rhs" [C][NEQ], "signal" [C],
- * "slots" [C][4] int32. */
+ * "primF" [F][2][NEQ], "face" [C][NS][NEQ] (reconstructed state of each cell
+ * face slot, faces in ascending id, 0 in slots past the cell's faces),
+ * "rhs" [C][NEQ], "signal" [C], "slots" [C][NS] int32 (-1 past the cell's
+ * faces), "nslots" int32 NS: 4 when every cell is a tetrahedron, else 6. */
 int orc_get(const orc *o, const char *name, void *out, int64_t bytes);
 /* Test hook: overwrite the stored residual used by orc_update. */
 int orc_set_rhs(orc *o, const double *rhs);
@@ -90,6 +91,9 @@ int orc_relax_cell(const fk_params *p, double *U, int *iters, int *tsat_iters);
  * iterations recorded, *aborted 1 when the solve ended on the abort path.
  * Returns the iteration count.  For the oracle's own tests. */
 int orc_chem_trace(const fk_params *p, double *U, double *rows, int max_rows, int *n_rows, int *aborted);
+/* Threads a parallel region of this build uses: omp_get_max_threads() in the
+ * OpenMP build, 1 in the serial one. */
+int orc_threads(void);
 
 #ifdef __cplusplus
 }

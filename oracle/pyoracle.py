@@ -86,6 +86,7 @@ def _load(path):
         "orc_fnt_press_rlx": (None, [P, _dp, _dp, _dp, ctypes.POINTER(ctypes.c_float),
                                      ctypes.POINTER(ctypes.c_float)]),
         "orc_gibbs": (ctypes.c_double, [P, ctypes.c_int, ctypes.c_double, ctypes.c_double]),
+        "orc_threads": (ctypes.c_int, []),
         "orc_chem_trace": (ctypes.c_int, [P, _dp, _dp, ctypes.c_int, ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_int)]),
     }
@@ -179,12 +180,19 @@ class Oracle:
         rhs = np.ascontiguousarray(rhs, dtype=np.float64)
         _chk(self.L.orc_set_rhs(self.h, _d(rhs)))
 
+    def nslots(self):
+        """Face slots per cell: 4 when every cell is a tetrahedron, else 6."""
+        out = np.zeros(1, dtype=np.int32)
+        _chk(self.L.orc_get(self.h, b"nslots", out.ctypes.data_as(ctypes.c_void_p), 4))
+        return int(out[0])
+
     def get(self, name):
         C, E = self.n_cells, self.neq
         F = len(self._keep[0])
+        S = self.nslots()
         shapes = {"grad": ((C, E, 3), np.float64), "theta": ((C, E), np.float64), "wmin": ((C, E), np.float64),
-                  "wmax": ((C, E), np.float64), "primF": ((F, 2, E), np.float64), "face": ((C, 4, E), np.float64),
-                  "rhs": ((C, E), np.float64), "signal": ((C,), np.float64), "slots": ((C, 4), np.int32)}
+                  "wmax": ((C, E), np.float64), "primF": ((F, 2, E), np.float64), "face": ((C, S, E), np.float64),
+                  "rhs": ((C, E), np.float64), "signal": ((C,), np.float64), "slots": ((C, S), np.int32)}
         shape, dt = shapes[name]
         out = np.empty(shape, dtype=dt)
         _chk(self.L.orc_get(self.h, name.encode(), out.ctypes.data_as(ctypes.c_void_p), out.nbytes))
@@ -292,3 +300,8 @@ def chem_trace(params, U, max_rows=256):
     n, ab = ctypes.c_int(), ctypes.c_int()
     it = lib().orc_chem_trace(ctypes.byref(_P(params)), _d(U), _d(rows), max_rows, ctypes.byref(n), ctypes.byref(ab))
     return U, it, rows[:n.value].copy(), bool(ab.value)
+
+
+def threads(omp=True):
+    """omp_get_max_threads() of the OpenMP build (1 for the serial one)."""
+    return lib(omp).orc_threads()

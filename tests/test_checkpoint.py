@@ -79,3 +79,33 @@ def test_restart_continues_bit_exactly(gpu, tmp_path, name):
         g.run_steps(3)
         assert np.array_equal(g.download_state(), ref)
         assert g.clock() == ref_clock
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["shock_bubble", "tgv", "mixed_elements_tp"])
+def test_restart_equals_the_oracles_uninterrupted_run(gpu, tmp_path, name):
+    """The restart anchored on the oracle: 3 steps on the device, checkpoint,
+    a new context loads it and runs 3 more steps; the state and the clock
+    equal the CPU oracle's 6 uninterrupted steps bit for bit (FP64, adaptive
+    dt every 2 steps, so the restored step counter decides the dt steps).
+    Replaces writeFieldRestart / readField (restart.cu:113-152, 424-497),
+    whose %.10e text is not bit-exact."""
+    from fooskagpu_amd import GpuSolver
+    from oracle.pyoracle import Oracle
+    from test_gpu_parity import SMALL
+    c = SMALL[name]()
+    p = dict(c.params, adaptDtItr=2)
+    o = Oracle(p, c.mesh, c.U)
+    dto, to = o.run_steps(6)
+    Uo = o.state()
+    with GpuSolver(p, c.mesh, c.U) as g:
+        g.run_steps(3)
+        checkpoint.save(tmp_path / "mid.ck", g)
+    with GpuSolver(p, c.mesh, c.U) as g:
+        checkpoint.load(tmp_path / "mid.ck", g)
+        dtg, tg = g.run_steps(3)
+        Ug = g.download_state()
+        step = g.clock()[2]
+    assert np.isfinite(Ug).all()
+    assert np.array_equal(Ug, Uo)
+    assert dtg == dto and tg == to and step == 6

@@ -38,7 +38,7 @@ def test_forward_matches_cpu_reference(gpu):
     yc = yn * norm[9:12] + norm[6:9]
     err = np.abs(yg - yc) / norm[9:12]
     print(f"forward: max |dy|/std {err.max():.2e}, mean {err.mean():.2e}")
-    assert err.max() < 2e-2 and err.mean() < 2e-3
+    assert err.max() < 5e-3 and err.mean() < 5e-4
 
 
 def test_backward_matches_cpu_reference(gpu):
@@ -55,11 +55,11 @@ def test_backward_matches_cpu_reference(gpu):
     rel = np.linalg.norm(gg - gc) / np.linalg.norm(gc)
     cos = gg @ gc / (np.linalg.norm(gg) * np.linalg.norm(gc))
     print(f"backward: relative L2 {rel:.2e}, cosine {cos:.6f}, loss gpu {loss:.6f} cpu {lc:.6f}")
-    assert rel < 2e-2 and cos > 0.9995
+    assert rel < 1e-5 and cos > 0.9999999
     assert loss == pytest.approx(lc, rel=1e-3)
     for name, (a, b) in {"W1": (0, 192), "W2": (256, 4352), "W3": (4416, 8512), "W4": (8576, 8768)}.items():
         r = np.linalg.norm(gg[a:b] - gc[a:b]) / np.linalg.norm(gc[a:b])
-        assert r < 3e-2, (name, r)
+        assert r < 1e-5, (name, r)
 
 
 def test_training_reduces_loss_and_is_deterministic(gpu):
@@ -187,3 +187,35 @@ def test_surrogate_classification_check_fails_before_any_stage(gpu):
         with pytest.raises(FkError, match="Cp_G"):
             g.run_steps(2)
         assert np.array_equal(g.download_state(), c.U) and g.clock()[2] == 0
+
+
+def test_network_halves_the_newton_work_on_chemical_cells(gpu):
+    """What the network adds where it is used: on chemical-class states
+    whose equilibrium exists (the plain reference solve reached p-T-g
+    equilibrium from them; the labelling keeps exactly those), the polish
+    from the MLP prediction needs at most 0.6x the Newton iterations of the
+    same polish from the cell state (FK_RELAX_COLD), falls back to the plain
+    solve for fewer than 5 % of the cells, and leaves at least 95 % of them
+    at p-T equilibrium."""
+    c, g = small_solver(n=20)
+    with g:
+        sg = surrogate.Surrogate(g)
+        x, y = surrogate.finite(*sg.label(surrogate.uniform_states(300000, c.params, seed=6)))
+        sg.set_data(x, y)
+        sg.set_weights(surrogate.init_weights(0), surrogate.fit_norm(x, y))
+        l0, l1 = sg.train(3000, batch=4096, lr=2e-3, seed=2)
+        # chemical states with an equilibrium: label, keep the finite ones
+        cand = surrogate.uniform_states(200000, c.params, seed=5)
+        _, yc = sg.label(cand)
+        U = cand[np.isfinite(yc).all(1)][:g.n_cells]
+        U = np.concatenate([U, np.tile(U[:1], (g.n_cells - len(U), 1))]) if len(U) < g.n_cells else U
+        out = {}
+        for name, mode in (("cold", _lib.RELAX_COLD), ("mlp", _lib.RELAX_MLP)):
+            g.upload_state(U)
+            out[name] = (g.relaxation(mode), g.download_state())
+    cold, mlp = out["cold"][0], out["mlp"][0]
+    print(f"training loss {l0:.4f} -> {l1:.6f}; cold {cold}\nmlp {mlp}")
+    assert mlp["n_chem"] == cold["n_chem"] and mlp["n_chem"] >= 0.99 * g.n_cells
+    assert mlp["newton_iters"] <= 0.6 * cold["newton_iters"]
+    assert mlp["n_fallback"] < 0.05 * mlp["n_chem"]
+    assert at_equilibrium(out["mlp"][1], c.params).mean() >= 0.95
