@@ -7,6 +7,14 @@ at stage 0) + the per-cell phase relaxation, on the 48M-tet shock-bubble
 workload (200^3 jittered hexes x 6 tets, 64 vapour bubbles, WALE on).
 Inputs are resident in HBM before the timed region.
 
+Legs at 48M cells: mixed precision with the surrogate relaxation (the
+headline), the same with the cell state in place of the network (cold
+start, what the network adds), mixed with the reference's plain relaxation,
+and FP64.  Errors against the FP64 CPU oracle come from the same legs run on
+the oracle's sample (40^3 x 6 cells, the steps the oracle ran).  A
+relaxation micro-benchmark times the three relaxations on far-from-
+equilibrium states, where chemical cells are common.
+
 Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 200]
 """
 import argparse
@@ -48,12 +56,15 @@ def relax_bytes(neq):
     return 8 * neq + 48
 
 
-def kernel_bytes(name, B, neq):
+def kernel_bytes(name, B, neq, mixed=True):
     """Algorithmic bytes per cell of one launch of kernel `name` (None when
-    the kernel has no per-cell byte model)."""
+    the kernel has no per-cell byte model).  pass_f_relax is the last RK
+    stage (U0 read) with the per-cell relaxation on the state in registers:
+    it adds the FP32 primitive record of each relaxed cell (4 NEQ, mixed)."""
     return {
         "pass_g": B["g"],
         "pass_f": B["f_u0"],          # fused stage update (stages 2..s read U0)
+        "pass_f_relax": B["f_u0"] + (4 * neq if mixed else 0),
         "pass_f_res": B["f_nou0"],    # residual written in place of U_new
         "pass_f_rhs": B["f_nou0"] + 8,  # + per-cell time-step bound
         "rk_update": 8 * neq * 4 + 8,  # U, U0, residual read, U_new written, V
@@ -107,20 +118,32 @@ def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None, 
         info["relaxation"] = {"newton_iters_per_cell": rt["newton_iters"] / cs,
                               "tsat_iters_per_cell": rt["tsat_iters"] / cs,
                               "chem_fraction": rt["n_chem"] / cs, "warm_fraction": rt["n_warm"] / cs,
-                              "fallback_fraction": rt["n_fallback"] / cs}
+                              "fallback_fraction": rt["n_fallback"] / cs, "n_chem": rt["n_chem"],
+                              "n_warm": rt["n_warm"]}
         log(f"relaxation: {info['relaxation']}")
     U = g.download_state()
     g.close()
     return el, kt, U, dt, info
 
 
+def rel_errors(U, Uo):
+    """Relative L1 of density, momentum, total energy and vapour fraction."""
+    rho, rho_o = U[:, 1] + U[:, 2], Uo[:, 1] + Uo[:, 2]
+    return {"density": float(np.abs(rho - rho_o).sum() / np.abs(rho_o).sum()),
+            "momentum": float(np.abs(U[:, 3:6] - Uo[:, 3:6]).sum() / max(np.abs(Uo[:, 3:6]).sum(), 1e-300)),
+            "energy": float(np.abs(U[:, 8] - Uo[:, 8]).sum() / np.abs(Uo[:, 8]).sum()),
+            "vapour_fraction": float(np.abs(Uo[:, 0] - U[:, 0]).sum() / np.abs(1 - Uo[:, 0]).sum())}
+
+
 def cpu_baseline(target_s=15.0):
     """FP64 oracle with OpenMP on a bounded sample of the same workload
-    (shock-bubble at 40^3 hexes x 6 = 384k cells), steps until ~target_s."""
+    (shock-bubble at 40^3 hexes x 6 = 384k cells), steps until ~target_s.
+    Returns the baseline and (case, steps, oracle state) for the error legs."""
     from fooskagpu_amd import cases
+    from oracle import pyoracle
     from oracle.pyoracle import Oracle
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    c = cases.shock_bubble(n=40, nbub=64)
+    threads = pyoracle.threads(omp=True)
+    c = cases.shock_bubble(n=40, nbub=64, order="morton")
     o = Oracle(c.params, c.mesh, c.U, omp=True)
     steps, el = 0, 0.0
     while el < target_s and steps < 50:
@@ -129,9 +152,73 @@ def cpu_baseline(target_s=15.0):
         el += time.perf_counter() - t0
         steps += 1
     v = c.mesh.n_cells * 3 * steps / el
-    return dict(value=v, unit="cell-RK-stage updates/s", cores=threads, kind="port",
-                sample=f"FP64 OpenMP oracle, shock-bubble 40^3x6 = {c.mesh.n_cells} cells, {steps} SSP-RK3 steps "
-                       f"incl. relaxation, {el:.1f} s")
+    base = dict(value=v, unit="cell-RK-stage updates/s", cores=threads, kind="port",
+                nproc=len(os.sched_getaffinity(0)),
+                sample=f"FP64 OpenMP oracle ({threads} threads), shock-bubble 40^3x6 = {c.mesh.n_cells} cells "
+                       f"in Morton order, {steps} SSP-RK3 steps incl. relaxation, {el:.1f} s")
+    return base, (c, steps, o.state())
+
+
+def cpu_configs():
+    """The OpenMP oracle (and the serial one on Sod) timed on the benchmark
+    configurations: Sod in full (100 steps), the 64^3 TGV and 128^3 bubble
+    meshes for a few steps each (their full runs take minutes on the CPU).
+    Rates in cell-RK-stage updates/s, setup excluded, nothing extrapolated."""
+    from fooskagpu_amd import cases
+    from oracle import pyoracle
+    from oracle.pyoracle import Oracle
+    out = {"omp_threads": pyoracle.threads(omp=True), "nproc": len(os.sched_getaffinity(0))}
+    for name, make, steps, omps in (("sod_32x8x8", lambda: cases.sod(), 100, (False, True)),
+                                    ("tgv_64", lambda: cases.tgv(n=64), 3, (True,)),
+                                    ("bubble_128", lambda: cases.bubble(n=128), 2, (True,))):
+        c = make()
+        for omp in omps:
+            o = Oracle(c.params, c.mesh, c.U, omp=omp)
+            t0 = time.perf_counter()
+            o.run_steps(steps)
+            el = time.perf_counter() - t0
+            key = f"{name}_{'omp' if omp else 'serial'}"
+            out[key] = {"cells": c.mesh.n_cells, "steps": steps, "seconds": el,
+                        "value": c.mesh.n_cells * 3 * steps / el}
+            log(f"cpu {key}: {out[key]['value']:.4g} cell-stage/s ({el:.1f} s)")
+            del o
+        del c
+    return out
+
+
+def relax_microbench(n=88):
+    """The three relaxations on far-from-equilibrium states (phasic pressures
+    and temperatures drawn independently, mixture densities 0.02 .. 1000
+    kg/m^3; about a third of the cells are chemical): plain reference Newton
+    solves, the surrogate pipeline from the cell state (cold) and with the
+    MLP warm start.  One fk_relaxation each on n^3 x 6 cells."""
+    from fooskagpu_amd import GpuSolver, RELAX_COLD, RELAX_MLP, RELAX_PLAIN, cases, surrogate
+    c = cases.bubble(n=n)
+    out = {"cells": c.mesh.n_cells}
+    with GpuSolver(c.params, c.mesh, c.U) as g:
+        sg = surrogate.Surrogate(g)
+        x, y = surrogate.finite(*sg.label(surrogate.uniform_states(1_000_000, c.params, seed=6)))
+        sg.set_data(x, y)
+        sg.set_weights(surrogate.init_weights(0), surrogate.fit_norm(x, y))
+        l0, l1 = sg.train(3000, batch=8192, lr=2e-3, seed=2)
+        out["training"] = {"labels": len(x), "loss_first": l0, "loss_last": l1}
+        U = surrogate.uniform_states(g.n_cells, c.params, seed=5)
+        for name, mode in (("plain", RELAX_PLAIN), ("cold", RELAX_COLD), ("mlp", RELAX_MLP)):
+            best = None
+            for _ in range(2):
+                g.upload_state(U)
+                g.sync()
+                t0 = time.perf_counter()
+                st = g.relaxation(mode)  # returns after reading the counts (a sync)
+                el = time.perf_counter() - t0
+                best = el if best is None else min(best, el)
+            out[name] = {"ms": best * 1e3, "newton_iters_per_cell": st["newton_iters"] / g.n_cells,
+                         "chem_fraction": st["n_chem"] / g.n_cells, "fallbacks": st["n_fallback"],
+                         "warm": st["n_warm"]}
+            log(f"relaxation micro-benchmark {name}: {out[name]}")
+    out["mlp_vs_cold_time"] = out["mlp"]["ms"] / out["cold"]["ms"]
+    out["mlp_vs_cold_iters"] = out["mlp"]["newton_iters_per_cell"] / out["cold"]["newton_iters_per_cell"]
+    return out
 
 
 def main():
@@ -140,9 +227,11 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=200, help="hexes per axis (200 -> 48M tets)")
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU legs (baseline, per-config timings)")
     ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--no-mlp", action="store_true", help="skip the surrogate run")
+    ap.add_argument("--no-mlp", action="store_true", help="skip the surrogate runs")
+    ap.add_argument("--no-extra", action="store_true",
+                    help="skip the cold-start leg, the oracle error legs and the relaxation micro-benchmark")
     ap.add_argument("--order", default="morton", choices=["rcm", "morton", "none"], help="cell order of the mesh")
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--stage-f", type=int, default=1, choices=[0, 1],
@@ -157,7 +246,7 @@ def main():
               "use --gpus 1", file=sys.stderr)
         sys.exit(2)
 
-    from fooskagpu_amd import FP64, MIXED, cases
+    from fooskagpu_amd import FP64, MIXED, RELAX_COLD, RELAX_MLP, RELAX_PLAIN, cases
     t0 = time.perf_counter()
     case = cases.shock_bubble(n=args.n, nbub=64, order=args.order)
     C = case.mesh.n_cells
@@ -172,13 +261,13 @@ def main():
             ms = min(g0.stream_copy(args.copy_bytes) for _ in range(3))
         copy = {"bytes": args.copy_bytes, "ms": ms, "GBps": 2 * args.copy_bytes / ms / 1e6}
         log(f"stream_copy {args.copy_bytes} B: {ms:.3f} ms, {copy['GBps']:.0f} GB/s (read + write)")
-    from fooskagpu_amd import RELAX_MLP, RELAX_PLAIN
     ntrain = min(4_000_000, C)
     train = dict(n_solver=ntrain, n_uniform=ntrain // 4, warm_steps=3, train_steps=args.train_steps)
-    plan = [("mixed_mlp", MIXED, RELAX_MLP, train), ("mixed", MIXED, RELAX_PLAIN, None),
-            ("fp64", FP64, RELAX_PLAIN, None)]
+    plan = [("mixed_mlp", MIXED, RELAX_MLP, train), ("mixed_cold", MIXED, RELAX_COLD, None),
+            ("mixed", MIXED, RELAX_PLAIN, None), ("fp64", FP64, RELAX_PLAIN, None)]
     for name, prec, mode, tr in plan:
-        if (name == "fp64" and args.no_fp64) or (name == "mixed_mlp" and args.no_mlp):
+        if (name == "fp64" and args.no_fp64) or (name.startswith("mixed_") and args.no_mlp) or \
+                (name == "mixed_cold" and args.no_extra):
             continue
         log(f"{name}: warmup {args.warmup}, timing {args.steps} steps")
         el, kt, U, dt, info = run_gpu(case, prec, args.steps, args.warmup, relax_mode=mode, train=tr, fuse=args.fuse,
@@ -202,6 +291,11 @@ def main():
     achieved = bpc * C / avg_f if avg_f > 0 else 0.0
     traffic = measured_traffic(dom, C)
     whole = (B["total"] * C * s * args.steps + relax_bytes(10) * C * args.steps) / mixed["el"]
+    # the pass F family (residual stage, fused stage, last stage with the relaxation)
+    fam = [k for k in ("pass_f_rhs", "pass_f", "pass_f_relax") if k in kt]
+    fam_bytes = sum(kernel_bytes(k, B, 10) * C * kt[k][0] for k in fam)
+    fam_s = sum(kt[k][1] for k in fam) / 1e3
+    has_chem = head == "mixed_mlp" and mixed["info"].get("relaxation", {}).get("n_warm", 0) > 0
     out = {
         "metric": "cell-RK-stage updates/s, 1 MI355X: mixed+surrogate vs FP64; % of HBM peak",
         "value": mixed["rate"],
@@ -217,22 +311,29 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"shock-bubble {args.n}^3 jittered hexes x 6 tets ({C} cells), 64 vapour bubbles, "
                                f"WALE, SSP-RK3, CFL 0.1, mixed precision, "
-                               + ("MLP warm start + FP64 Newton polish" if head == "mixed_mlp" else
-                                  "plain Newton relaxation"),
+                               + ("surrogate relaxation (exact classification, analytic thermal Newton, MLP warm "
+                                  "start + FP64 polish for chemical cells: "
+                                  + ("present" if has_chem else "none in the timed steps") + ")"
+                                  if head == "mixed_mlp" else "plain Newton relaxation"),
                    "cells": C, "faces": case.mesh.n_faces, "cell_order": args.order, "fuse_update": args.fuse,
                    "stage_f": args.stage_f, "fuse_relax": 1},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": dom, "bytes_per_cell": bpc, "avg_launch_ms": avg_f * 1e3},
+        "pass_f_family": {"kernels": fam, "achieved_GBps": fam_bytes / fam_s / 1e9 if fam_s else None,
+                          "frac": fam_bytes / fam_s / HBM_PEAK if fam_s else None,
+                          "bytes_per_cell": {k: kernel_bytes(k, B, 10) for k in fam}},
         "whole_step_hbm_frac": whole / HBM_PEAK,
         "stream_copy": copy,
     }
     if head == "mixed_mlp":
         out["surrogate_training"] = {k: v for k, v in mixed["info"].items() if k != "relaxation"}
     out["relaxation"] = {k: r["info"].get("relaxation") for k, r in res.items()}
-    for other in ("mixed",):
+    for other in ("mixed_cold", "mixed"):
         if other in res and other != head:
             out[other] = {"value": res[other]["rate"], "ms_per_step": res[other]["el"] / args.steps * 1e3}
+    if "mixed_cold" in res and head == "mixed_mlp":
+        out["speedup_network_vs_cold_start"] = mixed["rate"] / res["mixed_cold"]["rate"]
     if "fp64" in res:
         f64 = res["fp64"]
         Bd = stage_bytes(10, s, fc, 8, 8)
@@ -240,18 +341,34 @@ def main():
                        "whole_step_hbm_frac": (Bd["total"] * C * s * args.steps + relax_bytes(10) * C * args.steps)
                        / f64["el"] / HBM_PEAK}
         out["speedup_mixed_vs_fp64"] = mixed["rate"] / f64["rate"]
-        Uo, Um = f64["U"], mixed["U"]
-        rho_o, rho_m = Uo[:, 1] + Uo[:, 2], Um[:, 1] + Um[:, 2]
-        out["mixed_vs_fp64_gpu_rel_l1"] = {
-            "density": float(np.abs(rho_m - rho_o).sum() / np.abs(rho_o).sum()),
-            "momentum": float(np.abs(Um[:, 3:6] - Uo[:, 3:6]).sum() / max(np.abs(Uo[:, 3:6]).sum(), 1e-300)),
-            "energy": float(np.abs(Um[:, 8] - Uo[:, 8]).sum() / np.abs(Uo[:, 8]).sum()),
-            "vapour_fraction": float(np.abs(Um[:, 0] - Uo[:, 0]).sum() / np.abs(Uo[:, 0]).sum()),
-        }
+        out["mixed_vs_fp64_gpu_rel_l1"] = rel_errors(mixed["U"], f64["U"])
+    del res, case
     if not args.no_cpu:
         log("cpu baseline ...")
-        out["cpu_baseline"] = cpu_baseline()
-        log(f"cpu baseline {out['cpu_baseline']['value']:.4g} cell-stage/s")
+        base, (cs, osteps, Uo) = cpu_baseline()
+        out["cpu_baseline"] = base
+        log(f"cpu baseline {base['value']:.4g} cell-stage/s")
+        if not args.no_extra:
+            # the GPU legs on the oracle's sample, same steps: errors against the FP64 oracle
+            errs = {}
+            for name, prec, mode in (("mixed_mlp", MIXED, RELAX_MLP), ("mixed", MIXED, RELAX_PLAIN),
+                                     ("fp64", FP64, RELAX_PLAIN)):
+                if name == "mixed_mlp" and args.no_mlp:
+                    continue
+                tr = None
+                if name == "mixed_mlp":
+                    tr = dict(n_solver=cs.mesh.n_cells, n_uniform=cs.mesh.n_cells // 4, warm_steps=3,
+                              train_steps=args.train_steps)
+                _, _, Ug, _, _ = run_gpu(cs, prec, osteps, 0, relax_mode=mode, train=tr)
+                errs[name] = rel_errors(Ug, Uo)
+                log(f"{name} vs FP64 oracle on the sample: {errs[name]}")
+            out["vs_fp64_oracle_rel_l1"] = {"sample": base["sample"], **errs}
+            try:
+                out["cpu_configs"] = cpu_configs()
+            except MemoryError:
+                log("cpu configs skipped: out of host memory")
+    if not args.no_extra and not args.no_mlp:
+        out["relaxation_microbench"] = relax_microbench()
     print(json.dumps(out), flush=True)
 
 
