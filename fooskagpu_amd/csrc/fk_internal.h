@@ -86,6 +86,7 @@ struct fk_ctx {
   int64_t relax_calls = 0;  // relaxations summed in lcount[8, 13)
   // relaxation work list (chemical cells / surrogate list), one entry per cell
   uint32_t *rlist = nullptr;
+  uint32_t *rfall = nullptr;   // surrogate relaxation: cells whose polish failed
   unsigned long long *rpart = nullptr;  // per-workgroup relaxation counts [blocks][4]
   int64_t rlist_cap = 0;
   double dt = 0, time = 0;
@@ -93,9 +94,10 @@ struct fk_ctx {
   bool rhs_valid = false;
   bool rhs_f32 = false;      // c->rhs holds FP32 values (mixed dt stage, pass F MODE 1)
   bool fuse_update = true;   // RK update inside pass F (else residual + rk_update)
-  bool stage_f = true;       // pass F stages the workgroup's face states in LDS (mixed only)
+  bool stage_f = true;       // pass F stages the workgroup's face states in LDS (tetrahedral meshes)
   bool prims_w = true;       // mixed: producers of U also write wprim, pass G reads it
   bool fuse_relax = true;    // per-cell relaxation pass inside the last fused pass F
+  int64_t pf_waves = 0;      // waves of the last pass F launch (its per-wave relaxation counts)
   // FP32 primitive records [C][NEQ] of U[ia] (mixed precision), valid while
   // w_valid: set by the kernels that write them with a new state, cleared by
   // every other change of U

@@ -337,11 +337,21 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
 // just formed, so the state is not read back by a separate kernel: 1 runs
 // stage 1 of the surrogate relaxation (fk::screen_cell), 2 the plain
 // per-cell relaxation (fk::plain_cell).  Cells left for a second kernel go to
-// rlist (length in rcnt[5]); each wave stores its counts in rpart[wave]
+// rlist (1: fk::list_push, lengths in rcnt[5], rcnt[6]; 2: chemical cells,
+// length in rcnt[5]); each wave stores its counts in rpart[wave]
 // ([n_chem, iterations], and Tsat iterations for 2); Wout gets the primitive
 // records of the cells relaxed here.
+//
+// Workgroup size: 256 cells; 128 for FP64 with staging, whose face records
+// and gradients take 52 KB of LDS per 128 cells (104 KB at 256 would leave
+// one workgroup per CU).
+template <typename T, bool STAGE>
+constexpr int pass_f_block() {
+  return (STAGE && sizeof(T) == 8) ? 128 : kBlock;
+}
+
 template <int NEQ, typename T, bool VISC, bool HEAT, int MODE, bool STAGE, int NS, bool WOUT = false, int RLX = 0>
-__global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3)) pass_f(fk_params P, int64_t nc, const double *__restrict__ U,
+__global__ void __launch_bounds__((pass_f_block<T, STAGE>()), ((MODE == 1 || sizeof(T) == 8) ? 2 : 3)) pass_f(fk_params P, int64_t nc, const double *__restrict__ U,
                                                  const double *__restrict__ U0, double *__restrict__ Unew,
                                                  const int32_t *__restrict__ link,
                                                  const void *__restrict__ geo, const T *__restrict__ wls,
@@ -355,11 +365,12 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   static_assert(!RLX || (MODE == 0 && NEQ == 10), "fused relaxation: two-phase RK stage only");
   constexpr int KR = RLX == 2 ? 3 : 2;  // counts per wave in rpart
   constexpr int NGS = ngrad<NEQ, HEAT>();
-  const int64_t c0 = xcd_block() * kBlock;
+  constexpr int BS = pass_f_block<T, STAGE>();
+  const int64_t c0 = xcd_block() * BS;
   const int64_t c = c0 + threadIdx.x;
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
   T *sF = reinterpret_cast<T *>(lds_raw);
-  T *sG = sF + kBlock * NS * NEQ;
+  T *sG = sF + BS * NS * NEQ;
   if constexpr (STAGE) {
     // the workgroup's face records and gradients are contiguous in HBM and in
     // LDS: copied by direct-to-LDS loads (global_load_lds: lane i of a wave
@@ -367,27 +378,29 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
     // registers involved.  Measured 7.0 against 7.3 ms (fused pass F, 48M
     // cells) for a copy through registers, one cell per thread.
     typedef __attribute__((address_space(3))) void *lds_ptr;
-    const int64_t ncell = (nc - c0) < kBlock ? (nc - c0) : kBlock;
+    const int64_t ncell = (nc - c0) < BS ? (nc - c0) : BS;
     const int wbase = __builtin_amdgcn_readfirstlane((int)(threadIdx.x & ~63u));
     {
-      constexpr int kW = kBlock * NS * NEQ * (int)sizeof(T) / 16;  // 16-byte words of a full block
+      constexpr int kW = BS * NS * NEQ * (int)sizeof(T) / 16;  // 16-byte words of a full block
+      static_assert(kW % BS == 0, "whole 16-byte words per thread");
       const int nw = (int)(ncell * NS * NEQ * (int64_t)sizeof(T) / 16);
       const uint4 *src = reinterpret_cast<const uint4 *>(face + c0 * NS * NEQ);
       uint4 *dst = reinterpret_cast<uint4 *>(sF);
 #pragma unroll
-      for (int k = 0; k < kW / kBlock; ++k) {
-        const int i = k * kBlock + (int)threadIdx.x;
-        if (i < nw) __builtin_amdgcn_global_load_lds((const void *)(src + i), (lds_ptr)(dst + k * kBlock + wbase), 16, 0, 0);
+      for (int k = 0; k < kW / BS; ++k) {
+        const int i = k * BS + (int)threadIdx.x;
+        if (i < nw) __builtin_amdgcn_global_load_lds((const void *)(src + i), (lds_ptr)(dst + k * BS + wbase), 16, 0, 0);
       }
     }
     if constexpr (VISC) {
-      const int nw = (int)(ncell * NGS);
+      constexpr int WPC = NGS * (int)sizeof(T) / 4;  // 4-byte words per cell
+      const int nw = (int)(ncell * WPC);
       const float *src = reinterpret_cast<const float *>(gvel + c0 * NGS);
       float *dst = reinterpret_cast<float *>(sG);
 #pragma unroll
-      for (int k = 0; k < NGS; ++k) {
-        const int i = k * kBlock + (int)threadIdx.x;
-        if (i < nw) __builtin_amdgcn_global_load_lds((const void *)(src + i), (lds_ptr)(dst + k * kBlock + wbase), 4, 0, 0);
+      for (int k = 0; k < WPC; ++k) {
+        const int i = k * BS + (int)threadIdx.x;
+        if (i < nw) __builtin_amdgcn_global_load_lds((const void *)(src + i), (lds_ptr)(dst + k * BS + wbase), 4, 0, 0);
       }
     }
     // the direct-to-LDS loads count as vector memory loads: wait for all of
@@ -406,11 +419,11 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
   }
   // face state of global slot `sl` of cell `cell`, from LDS when staged
   auto slot_ptr = [&](int64_t cell, int64_t sl) -> const T * {
-    if (STAGE && cell >= c0 && cell < c0 + kBlock) return sF + (sl - c0 * NS) * NEQ;
+    if (STAGE && cell >= c0 && cell < c0 + BS) return sF + (sl - c0 * NS) * NEQ;
     return face + sl * NEQ;
   };
   auto grad_ptr = [&](int64_t cell) -> const T * {
-    if (STAGE && cell >= c0 && cell < c0 + kBlock) return sG + (cell - c0) * NGS;
+    if (STAGE && cell >= c0 && cell < c0 + BS) return sG + (cell - c0) * NGS;
     return gvel + cell * NGS;
   };
   const fk::Mat<double> md = fk::make_mat<double>(P);
@@ -678,18 +691,10 @@ __global__ void __launch_bounds__(kBlock, ((MODE == 1 || sizeof(T) == 8) ? 2 : 3
                      "+s"(mw.eta[1]), "+s"(mw.avfMin));
         fk::store_prims<NEQ>(mw, un, Wout + c * NEQ);
       }
-      // wave-aggregated append to the work list (its order does not affect
-      // results), and the wave's counts without atomics: it <= 4, so three
-      // ballots give the iteration sum
+      // wave-aggregated append to the work list, and the wave's counts
+      // without atomics: it <= 4, so three ballots give the iteration sum
       const int l = threadIdx.x & 63;
-      const unsigned long long mp = __ballot(push);
-      if (mp) {
-        const int leader = __ffsll((long long)mp) - 1;
-        unsigned long long b = 0;
-        if (l == leader) b = atomicAdd(&rcnt[5], (unsigned long long)__popcll(mp));
-        b = __shfl(b, leader, 64);
-        if (push) rlist[b + __popcll(mp & ((1ull << l) - 1ull))] = (uint32_t)((c << 1) | chem);
-      }
+      fk::list_push(push, chem, c, rlist, nc, rcnt);
       static_assert(fk::kColdIters < 8, "iteration counts are summed from three ballots");
       const unsigned long long nch = __popcll(__ballot(chem));
       const unsigned long long nit = __popcll(__ballot(it & 1)) + 2ull * __popcll(__ballot(it & 2)) +
@@ -977,7 +982,7 @@ int upload_geometry(fk_ctx *c) {
 void free_all(fk_ctx *c) {
   void *ptrs[] = {c->link, c->geo, c->rvec, c->wls, c->vol, c->volcfl, c->U[0], c->U[1], c->U[2], c->face,
                   c->gvel, c->rhs, c->dtc, c->red, c->dscal, c->dgrad, c->dtheta, c->lcount, c->rlist,
-                  c->rpart, c->wprim, c->health};
+                  c->rfall, c->rpart, c->wprim, c->health};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->link = nullptr;
@@ -985,6 +990,7 @@ void free_all(fk_ctx *c) {
   c->health = nullptr;
   c->w_valid = false;
   c->rlist = nullptr;
+  c->rfall = nullptr;
   c->rpart = nullptr;
   c->rlist_cap = 0;
   c->geo = c->rvec = c->wls = nullptr;
@@ -1049,10 +1055,14 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
   // fused relaxation (last stage): records of the relaxed state, mixed precision
   float *Wr = (sizeof(T) == 4 && c->prims_w) ? c->wprim : nullptr;
   unsigned long long *cnt = (unsigned long long *)c->lcount;
-  // LDS staging of face states (mixed precision, tetrahedral meshes: 40 KB +
-  // 9 KB per workgroup)
-  const bool stage = c->stage_f && sizeof(T) == 4 && NS == 4;
-  const size_t lds = stage ? (size_t)kBlock * (NS * c->neq + 12) * sizeof(T) : 0;
+  // LDS staging of face states and gradients (tetrahedral meshes): mixed
+  // precision 256 cells, 49 KB per workgroup; FP64 128 cells, 52 KB
+  const bool stage = c->stage_f && NS == 4;
+  constexpr int bs = pass_f_block<T, true>();
+  const unsigned gs = (unsigned)(((c->nc + bs - 1) / bs + kXcd - 1) / kXcd * kXcd);
+  const size_t lds = stage ? (size_t)bs * (NS * c->neq + 12) * sizeof(T) : 0;
+  // waves of this launch (per-wave relaxation counts of the fused last stage)
+  c->pf_waves = (int64_t)(stage ? (int64_t)gs * (bs / 64) : (int64_t)gr * (kBlock / 64));
 #define PF_ARGS(W) \
   c->p, c->nc, U, U0, Unew, c->link, c->geo, wl, c->vol, c->volcfl, face, gv, c->rhs, c->dtc, irk, dt, dtp, W
 #define PF(NEQ, VISC, HEAT)                                                                                            \
@@ -1061,7 +1071,7 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
       if (relax == 1) {                                                                                                \
         if (stage)                                                                                                     \
           LAUNCH_TIMED(c, "pass_f_relax",                                                                              \
-                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, (NS == 4), NS, false, 1>), dim3(gr), dim3(kBlock),   \
+                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, (NS == 4), NS, false, 1>), dim3(gs), dim3(bs),       \
                                           lds, c->stream, PF_ARGS(Wr), c->rlist, cnt, c->rpart));                      \
         else                                                                                                           \
           LAUNCH_TIMED(c, "pass_f_relax",                                                                              \
@@ -1072,7 +1082,7 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
       if (relax == 2) {                                                                                                \
         if (stage)                                                                                                     \
           LAUNCH_TIMED(c, "pass_f_relax",                                                                              \
-                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, (NS == 4), NS, false, 2>), dim3(gr), dim3(kBlock),   \
+                       hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, (NS == 4), NS, false, 2>), dim3(gs), dim3(bs),       \
                                           lds, c->stream, PF_ARGS(Wr), c->rlist, cnt, c->rpart));                      \
         else                                                                                                           \
           LAUNCH_TIMED(c, "pass_f_relax",                                                                              \
@@ -1084,8 +1094,8 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
     if constexpr (MODE == 0 && sizeof(T) == 4) {                                                                       \
       if (Wout) {                                                                                                      \
         if (stage)                                                                                                     \
-          LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, (NS == 4), NS, true>), dim3(gr),             \
-                                                 dim3(kBlock), lds, c->stream, PF_ARGS(Wout), nullptr, nullptr,         \
+          LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, (NS == 4), NS, true>), dim3(gs),             \
+                                                 dim3(bs), lds, c->stream, PF_ARGS(Wout), nullptr, nullptr,             \
                                                  nullptr));                                                             \
         else                                                                                                           \
           LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false, NS, true>), dim3(gr),            \
@@ -1095,7 +1105,7 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
       }                                                                                                                \
     }                                                                                                                  \
     if (stage)                                                                                                         \
-      LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, (NS == 4), NS>), dim3(gr), dim3(kBlock), lds,     \
+      LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, (NS == 4), NS>), dim3(gs), dim3(bs), lds,         \
                                              c->stream, PF_ARGS(Wout), nullptr, nullptr, nullptr));                     \
     else                                                                                                               \
       LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false, NS>), dim3(gr), dim3(kBlock), 0,      \
@@ -1237,11 +1247,14 @@ int ensure_rlist(fk_ctx *c) {
   if (c->nc >= (int64_t(1) << 31)) return fail(FK_ERANGE, "relaxation supports < 2^31 cells");
   if (c->rlist_cap >= c->nc) return FK_OK;
   if (c->rlist) (void)hipFree(c->rlist);
+  if (c->rfall) (void)hipFree(c->rfall);
   if (c->rpart) (void)hipFree(c->rpart);
   c->rlist = nullptr;
+  c->rfall = nullptr;
   c->rpart = nullptr;
   c->rlist_cap = 0;
   HIPCHK(hipMalloc(&c->rlist, (size_t)c->nc * sizeof(uint32_t)));
+  HIPCHK(hipMalloc(&c->rfall, (size_t)c->nc * sizeof(uint32_t)));
   // [blocks][4] partials of the relaxation kernels, or [waves][2 or 3] of the
   // fused pass F (grid_xcd blocks of 4 waves)
   HIPCHK(hipMalloc(&c->rpart, (size_t)grid_xcd(c->nc) * 12 * sizeof(unsigned long long)));
@@ -1310,7 +1323,7 @@ int do_relax(fk_ctx *c, fk_relax_stats *st, int mode = FK_RELAX_PLAIN) {
 // counts, stage 2 and the running totals (as do_relax).
 int relax_fused_finish(fk_ctx *c, bool warm) {
   unsigned long long *cnt = (unsigned long long *)c->lcount;
-  const int64_t nw = (int64_t)grid_xcd(c->nc) * (kBlock / 64);
+  const int64_t nw = c->pf_waves;
   LAUNCH_TIMED(c, "relax_count", hipLaunchKernelGGL(fk::reduce_partials<2>, dim3(fk::reduce_grid(nw)), dim3(256), 0,
                                                     c->stream, c->rpart, nw, cnt));
   int rc = fk_relax_mlp_finish(c, warm);
@@ -1334,7 +1347,7 @@ int relax_plain_begin(fk_ctx *c) {
 
 int relax_plain_fused_finish(fk_ctx *c) {
   unsigned long long *cnt = (unsigned long long *)c->lcount;
-  const int64_t nw = (int64_t)grid_xcd(c->nc) * (kBlock / 64);
+  const int64_t nw = c->pf_waves;
   LAUNCH_TIMED(c, "relax_count", hipLaunchKernelGGL(fk::reduce_partials<3>, dim3(fk::reduce_grid(nw)), dim3(256), 0,
                                                     c->stream, c->rpart, nw, cnt));
   float *W = (c->p.precision == FK_MIXED && c->prims_w) ? c->wprim : nullptr;

@@ -36,6 +36,7 @@ namespace {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
 
 constexpr int HID = 64;
 constexpr int kParams = 8771;
@@ -67,12 +68,21 @@ struct FwdLds {
   float y[16 * 3];
 };
 
+// The FP32 parameters of the forward pass, copied into LDS once per block:
+// layer 1 as (W1 row k, b1[k]) quads, the biases of layers 2 and 3, W4, b4.
+struct SmallW {
+  float4 w1b[HID];
+  float b2[HID], b3[HID];
+  __attribute__((aligned(16))) float w4[3 * HID];
+  float b4[4];
+};
+
 // Forward of 16 samples by one wave.  x: LDS [16][3] (normalised), result in
-// lds.y [16][3] (normalised).  Weights: fp32 w (W1, b1, b2, b3, W4, b4),
+// lds.y [16][3] (normalised).  Weights: sw (LDS: W1, b1, b2, b3, W4, b4),
 // bf16 wb in LDS (W2, W3).  Keeps the BF16 activations of each layer in hs
 // (row-major [16][64]) when hs != nullptr (training).
-__device__ __forceinline__ void mlp_fwd16(const float *__restrict__ w, const uint16_t *wb, FwdLds &lds,
-                                         uint16_t *h1s, uint16_t *h2s, uint16_t *h3s) {
+__device__ __forceinline__ void mlp_fwd16(const SmallW &sw, const uint16_t *wb, FwdLds &lds, uint16_t *h1s,
+                                         uint16_t *h2s, uint16_t *h3s) {
   const int l = threadIdx.x & 63;
   const int m = l & 15, q = l >> 4;
   // layer 1 (FP32): this lane computes neurons k = 32 ks + 8 q + i of sample m
@@ -83,7 +93,8 @@ __device__ __forceinline__ void mlp_fwd16(const float *__restrict__ w, const uin
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
       const int k = 32 * ks + 8 * q + i;
-      const float z = w[oB1 + k] + w[oW1 + 3 * k] * x0 + w[oW1 + 3 * k + 1] * x1 + w[oW1 + 3 * k + 2] * x2;
+      const float4 c = sw.w1b[k];
+      const float z = c.w + c.x * x0 + c.y * x1 + c.z * x2;
       const uint16_t hb = f2bf(tanhf(z));
       a[ks][i] = __builtin_bit_cast(__bf16, hb);
       if (h1s) h1s[m * HID + k] = hb;
@@ -92,7 +103,7 @@ __device__ __forceinline__ void mlp_fwd16(const float *__restrict__ w, const uin
 #pragma unroll
   for (int layer = 0; layer < 2; ++layer) {
     const uint16_t *W = wb + (layer == 0 ? bW2 : bW3);
-    const float *bias = w + (layer == 0 ? oB2 : oB3);
+    const float *bias = layer == 0 ? sw.b2 : sw.b3;
     uint16_t *hs = layer == 0 ? h2s : h3s;
 #pragma unroll
     for (int nt = 0; nt < 4; ++nt) {
@@ -117,16 +128,45 @@ __device__ __forceinline__ void mlp_fwd16(const float *__restrict__ w, const uin
     for (int ks = 0; ks < 2; ++ks) a[ks] = ld_bf8(lds.h + m * HID + 32 * ks + 8 * q);
     __builtin_amdgcn_wave_barrier();
   }
-  // layer 4 (FP32) on the last activations, still in lds.h
-  if (l < 48) {
-    const int j = q;
-    float s = w[oB4 + j];
-#pragma unroll 8
-    for (int k = 0; k < HID; ++k) s += w[oW4 + j * HID + k] * bf2f(lds.h[m * HID + k]);
-    lds.y[m * 3 + j] = s;
+  // layer 4 (FP32) on the last activations: this lane's 16 neurons of sample
+  // m (in a[]), then the sum over the sample's 4 lanes q = 0..3,
+  // (p_0 + p_1) + (p_2 + p_3) on every one of them
+  float p[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int ks = 0; ks < 2; ++ks) {
+    // the whole vector reinterpreted (a bit_cast of one element of a[ks]
+    // reads element 0)
+    const u16x8 ab = __builtin_bit_cast(u16x8, a[ks]);
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const float h = bf2f(ab[i]);
+      const int k = 32 * ks + 8 * q + i;
+#pragma unroll
+      for (int j = 0; j < 3; ++j) p[j] += sw.w4[j * HID + k] * h;
+    }
   }
+#pragma unroll
+  for (int j = 0; j < 3; ++j) {
+    p[j] += __shfl_xor(p[j], 16, 64);
+    p[j] += __shfl_xor(p[j], 32, 64);
+  }
+  if (q == 0)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) lds.y[m * 3 + j] = sw.b4[j] + p[j];
   __builtin_amdgcn_s_waitcnt(0);
   __builtin_amdgcn_wave_barrier();
+}
+
+// Copy the FP32 forward parameters into LDS (block-wide; the caller's
+// barrier, that of load_wb, publishes them).
+__device__ __forceinline__ void load_small(const float *__restrict__ w, SmallW &s) {
+  for (int i = threadIdx.x; i < HID; i += blockDim.x) {
+    s.w1b[i] = make_float4(w[oW1 + 3 * i], w[oW1 + 3 * i + 1], w[oW1 + 3 * i + 2], w[oB1 + i]);
+    s.b2[i] = w[oB2 + i];
+    s.b3[i] = w[oB3 + i];
+  }
+  for (int i = threadIdx.x; i < 3 * HID; i += blockDim.x) s.w4[i] = w[oW4 + i];
+  if (threadIdx.x < 3) s.b4[threadIdx.x] = w[oB4 + threadIdx.x];
 }
 
 // Copy the BF16 weights into LDS (block-wide).
@@ -143,6 +183,8 @@ __global__ void __launch_bounds__(256) mlp_infer_k(int64_t n, const float *__res
                                                    Norm nm) {
   __shared__ __attribute__((aligned(16))) uint16_t wb[8192];
   __shared__ FwdLds lds[4];
+  __shared__ SmallW sw;
+  load_small(w, sw);
   load_wb(wbg, wb, 8192);
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
   const int64_t nchunk = (n + 63) / 64;
@@ -156,7 +198,7 @@ __global__ void __launch_bounds__(256) mlp_infer_k(int64_t n, const float *__res
       }
       __builtin_amdgcn_s_waitcnt(0);
       __builtin_amdgcn_wave_barrier();
-      mlp_fwd16(w, wb, lds[wv], nullptr, nullptr, nullptr);
+      mlp_fwd16(sw, wb, lds[wv], nullptr, nullptr, nullptr);
       if (l < 48) {
         const int mm = l / 3, j = l % 3;
         const int64_t s = base + mm;
@@ -187,6 +229,8 @@ __global__ void __launch_bounds__(64) mlp_train_k(int64_t B, const int64_t *__re
                                                   Norm nm, float *__restrict__ part, float *__restrict__ loss_part) {
   __shared__ __attribute__((aligned(16))) uint16_t wb[kBf];
   __shared__ __attribute__((aligned(16))) TrainLds L;
+  __shared__ SmallW sw;
+  load_small(w, sw);
   load_wb(wbg, wb, kBf);
   const int l = threadIdx.x, m16 = l & 15, q = l >> 4;
   float *g = part + (int64_t)blockIdx.x * kParams;
@@ -205,7 +249,7 @@ __global__ void __launch_bounds__(64) mlp_train_k(int64_t B, const int64_t *__re
     if (l < 48) L.f.x[l] = L.x[t * 48 + l];
     __builtin_amdgcn_s_waitcnt(0);
     __builtin_amdgcn_wave_barrier();
-    mlp_fwd16(w, wb, L.f, L.h1 + t * 16 * HID, L.h2 + t * 16 * HID, L.h3 + t * 16 * HID);
+    mlp_fwd16(sw, wb, L.f, L.h1 + t * 16 * HID, L.h2 + t * 16 * HID, L.h3 + t * 16 * HID);
     if (l < 48) {
       const int mm = l / 3, j = l % 3;
       const int64_t s = s0 + t * 16 + mm;
@@ -382,7 +426,9 @@ __global__ void __launch_bounds__(256) mlp_label_k(fk_params P, int64_t n, const
 // thermal cells that did not converge go to the work list as
 // (cell << 1) | chem for stage 2.  The last RK stage's fused pass F runs the
 // same stage 1 on the state it produces (fk_kernels.hip).
-// counts: [n_chem, newton iterations, -, fallbacks, warm-started, list length].
+// counts: [n_chem, newton iterations, -, fallbacks, warm-started, list
+// length from the front (chemical), list length from the back (thermal),
+// fallback list length].
 
 __global__ void __launch_bounds__(256) relax_screen_k(fk_params P, int64_t nc, double *__restrict__ U,
                                                       uint32_t *__restrict__ list,
@@ -404,18 +450,7 @@ __global__ void __launch_bounds__(256) relax_screen_k(fk_params P, int64_t nc, d
     // relax_list_k)
     if (W && !push) fk::store_prims<10>(fk::make_mat<double>(P), u, W + c * 10);
   }
-  // wave-aggregated append to the list (its order does not affect results:
-  // every cell is solved independently)
-  const unsigned long long mask = __ballot(push);
-  const int npush = __popcll(mask);
-  unsigned long long base = 0;
-  const int leader = npush ? __ffsll((long long)mask) - 1 : 0;
-  if (npush && l == leader) base = atomicAdd(&counts[5], (unsigned long long)npush);
-  base = __shfl(base, leader, 64);
-  if (push) {
-    const int rank = __popcll(mask & ((1ull << l) - 1ull));
-    list[base + rank] = (uint32_t)((c << 1) | chem);
-  }
+  fk::list_push(push, chem, c, list, nc, counts);
   unsigned long long v[2] = {(unsigned long long)chem, (unsigned long long)it};
   fk::block_partials<2>(v, part);  // [n_chem, newton iterations]
 }
@@ -428,28 +463,34 @@ template <bool WARM>
 __global__ void __launch_bounds__(256) relax_list_k(fk_params P, double *__restrict__ U,
                                                     const uint32_t *__restrict__ list, const float *__restrict__ w,
                                                     const uint16_t *__restrict__ wbg, Norm nm,
-                                                    unsigned long long *__restrict__ counts, float *__restrict__ W) {
+                                                    unsigned long long *__restrict__ counts, float *__restrict__ W,
+                                                    int64_t cap, uint32_t *__restrict__ fall) {
   __shared__ __attribute__((aligned(16))) uint16_t wb[8192];
   __shared__ FwdLds lds[4];
-  const int64_t n = (int64_t)counts[5];
-  const int64_t nchunk = (n + 63) / 64;
+  __shared__ SmallW sw;
+  // chunks of 64 entries: the chemical ones from the list's front, then the
+  // thermal ones from its back
+  const int64_t nfr = (int64_t)counts[5], nbk = (int64_t)counts[6];
+  const int64_t cf = (nfr + 63) / 64, nchunk = cf + (nbk + 63) / 64;
   if ((int64_t)blockIdx.x * 4 >= nchunk) return;  // whole block idle: skip the weight load
-  if (WARM) load_wb(wbg, wb, 8192);
+  if (WARM && nfr > 0) {
+    load_small(w, sw);
+    load_wb(wbg, wb, 8192);
+  }
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
-  unsigned long long nit = 0, nfb = 0, nwarm = 0;
+  unsigned long long nit = 0, nwarm = 0;
   const int64_t per = (nchunk + 3) / 4 * 4;
   for (int64_t ch = (int64_t)blockIdx.x * 4 + wv; ch < per; ch += (int64_t)gridDim.x * 4) {
-    const int64_t e = ch * 64 + l;
-    const bool live = e < n;
-    const uint32_t ent = live ? list[e] : 0u;
+    const bool front = ch < cf;  // wave-uniform
+    const int64_t e = (front ? ch : ch - cf) * 64 + l;
+    const bool live = e < (front ? nfr : nbk);
+    const uint32_t ent = live ? list[front ? e : cap - 1 - e] : 0u;
     const int64_t c = (int64_t)(ent >> 1);
     const int chem = (int)(ent & 1u);
     double u[10];
     for (int v = 0; v < 10; ++v) u[v] = live ? U[c * 10 + v] : (v == 0 ? 0.5 : (v == 1 || v == 2 ? 1.0 : 0.0));
-    const double a0 = fk::clampd(P, u[0]);
-    const int pm = (1 - a0 > a0) ? 1 : 0, k = 1 - pm;
-    double as, rk, ek;  // start (alpha_k, rho_k, e_k)
-    if (WARM) {  // wave-uniform: every lane joins the cooperative inference
+    double T0 = 0, p0 = 0;  // chemical cells: start (T, p)
+    if (WARM && front) {  // wave-uniform: every lane joins the cooperative inference
       float xf[3];
       fk::mlp_features(P, u, xf);
       float yw[3];
@@ -458,54 +499,91 @@ __global__ void __launch_bounds__(256) relax_list_k(fk_params P, double *__restr
           for (int j = 0; j < 3; ++j) lds[wv].x[(l & 15) * 3 + j] = (xf[j] - nm.xm[j]) / nm.xs[j];
         __builtin_amdgcn_s_waitcnt(0);
         __builtin_amdgcn_wave_barrier();
-        mlp_fwd16(w, wb, lds[wv], nullptr, nullptr, nullptr);
+        mlp_fwd16(sw, wb, lds[wv], nullptr, nullptr, nullptr);
         if ((l >> 4) == t)
           for (int j = 0; j < 3; ++j) yw[j] = lds[wv].y[(l & 15) * 3 + j] * nm.ys[j] + nm.ym[j];
         __builtin_amdgcn_wave_barrier();
       }
-      // (T*, p*, alpha_L*) -> (alpha_k, rho_k(p*, T*), e_k(p*, rho_k))
-      const double Ts = (double)yw[0], ps = (double)kPScale * sinh((double)yw[1]);
-      const double aL = fmin(fmax((double)yw[2], 1e-12), 1.0 - 1e-12);
-      as = k == 0 ? aL : 1 - aL;
-      rk = (ps + P.pInf[k]) / ((P.Cp[k] / P.gamma[k]) * Ts * (P.gamma[k] - 1));
-      ek = (ps + P.gamma[k] * P.pInf[k] + (P.gamma[k] - 1) * P.eta[k] * rk) / (rk * (P.gamma[k] - 1));
-    }
-    if (!WARM || !chem) {  // thermal cells (and the ablation): the cell state
-      as = k == 0 ? a0 : 1 - a0;
-      rk = k == 0 ? u[1] / a0 : u[2] / (1 - a0);
-      if (chem) {
-        ek = k == 0 ? u[6] / u[1] : u[7] / u[2];
-      } else {
-        fk::ThermProb t;
-        fk::therm_setup(P, u, t);
-        ek = t.enr[k];
-      }
+      // predicted equilibrium (T*, p*); the alpha_L* output is not needed:
+      // the polish takes alpha_L from the mass balance at (p, T)
+      T0 = (double)yw[0];
+      p0 = (double)kPScale * sinh((double)yw[1]);
     }
     if (!live) continue;
+    if (chem && !WARM) fk::chem_cold(P, u, T0, p0);
     nwarm += (WARM && chem) ? 1 : 0;
-    double uu[10];
-    for (int v = 0; v < 10; ++v) uu[v] = u[v];
-    int it = chem ? fk::chem_polish(P, uu, as, rk, ek, P.polishTol, 30)
-                  : fk::therm_polish(P, uu, rk, ek, P.polishTol, 30);
-    if (it < 0) {  // fallback: the reference solve from the cell state
-      ++nfb;
-      nit += 30;
-      for (int v = 0; v < 10; ++v) uu[v] = u[v];
-      it = chem ? fk::chem_dev(P, uu) : fk::therm_dev(P, uu);
+    int it;
+    if (chem) {
+      it = fk::chem_polish(P, u, T0, p0, P.polishTol, 30);
+    } else {  // thermal cells: the cell state
+      fk::ThermProb t;
+      fk::therm_setup(P, u, t);
+      double rk, ek;
+      fk::therm_cold(P, u, t, rk, ek);
+      it = fk::therm_polish(P, u, rk, ek, P.polishTol, 30);
     }
+    // a failed polish leaves U as it was: the cell goes to the fallback list
+    // (relax_fallback_k runs the reference solve from the cell state)
+    const bool failed = it < 0;
+    const unsigned long long mf = __ballot(failed);
+    if (mf) {
+      const int leader = __ffsll((long long)mf) - 1;
+      unsigned long long b = 0;
+      if (l == leader) b = atomicAdd(&counts[7], (unsigned long long)__popcll(mf));
+      b = __shfl(b, leader, 64);
+      if (failed) fall[b + __popcll(mf & ((1ull << l) - 1ull))] = ent;
+    }
+    if (failed) continue;
     nit += it;
-    for (int v = 0; v < 10; ++v) U[c * 10 + v] = uu[v];
-    if (W) fk::store_prims<10>(fk::make_mat<double>(P), uu, W + c * 10);
+    U[c * 10 + 0] = u[0];
+    U[c * 10 + 1] = u[1];
+    U[c * 10 + 2] = u[2];
+    U[c * 10 + 6] = u[6];
+    U[c * 10 + 7] = u[7];
+    U[c * 10 + 9] = u[9];
+    if (W) fk::store_prims<10>(fk::make_mat<double>(P), u, W + c * 10);
   }
   for (int off = 32; off > 0; off >>= 1) {
     nit += __shfl_down(nit, off, 64);
-    nfb += __shfl_down(nfb, off, 64);
     nwarm += __shfl_down(nwarm, off, 64);
   }
   if (l == 0) {
     atomicAdd(&counts[1], nit);
-    atomicAdd(&counts[3], nfb);
     atomicAdd(&counts[4], nwarm);
+  }
+}
+
+// Fallback of the surrogate relaxation: the reference solve (thermRlx /
+// chemRlx, as the plain relaxation) from the cell state for the cells whose
+// polish failed (fall[0 .. counts[7]), entries (cell << 1) | chem).  Apart
+// from relax_list_k so that the reference solves' registers do not limit the
+// occupancy of the polish.  A fallback counts 30 iterations plus those of the
+// reference solve.
+__global__ void __launch_bounds__(256) relax_fallback_k(fk_params P, double *__restrict__ U,
+                                                        const uint32_t *__restrict__ fall,
+                                                        unsigned long long *__restrict__ counts,
+                                                        float *__restrict__ W) {
+  const int64_t n = (int64_t)counts[7];
+  unsigned long long nit = 0, nfb = 0;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    const uint32_t ent = fall[e];
+    const int64_t c = (int64_t)(ent >> 1);
+    double u[10];
+    for (int v = 0; v < 10; ++v) u[v] = U[c * 10 + v];
+    const int it = (ent & 1u) ? fk::chem_dev(P, u) : fk::therm_dev(P, u);
+    nit += 30 + (unsigned long long)it;
+    ++nfb;
+    for (int v = 0; v < 10; ++v) U[c * 10 + v] = u[v];
+    if (W) fk::store_prims<10>(fk::make_mat<double>(P), u, W + c * 10);
+  }
+  const int l = threadIdx.x & 63;
+  for (int off = 32; off > 0; off >>= 1) {
+    nit += __shfl_down(nit, off, 64);
+    nfb += __shfl_down(nfb, off, 64);
+  }
+  if (l == 0 && nfb) {
+    atomicAdd(&counts[1], nit);
+    atomicAdd(&counts[3], nfb);
   }
 }
 
@@ -605,7 +683,7 @@ int ensure_data(fk_ctx *c, int64_t cap) {
 }  // namespace
 
 // called from the relaxation entry point of fk_kernels.hip; counts layout as
-// relax_screen_k / relax_list_k (slots [0, 6) of c->lcount, zeroed here).
+// relax_screen_k / relax_list_k (slots [0, 8) of c->lcount, zeroed here).
 // Checks, scratch and zeroed counts of the surrogate relaxation (before stage
 // 1, which relax_screen_k or the last stage's fused pass F runs).
 int fk_relax_mlp_begin(fk_ctx *c, bool warm) {
@@ -618,7 +696,7 @@ int fk_relax_mlp_begin(fk_ctx *c, bool warm) {
   int rc = ensure_rlist(c);
   if (rc) return rc;
   unsigned long long *cnt = (unsigned long long *)c->lcount;
-  LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_u64_k, dim3(1), dim3(64), 0, c->stream, cnt, 6));
+  LAUNCH_TIMED(c, "zero", hipLaunchKernelGGL(zero_u64_k, dim3(1), dim3(64), 0, c->stream, cnt, 8));
   return FK_OK;
 }
 
@@ -645,10 +723,15 @@ int fk_relax_mlp_finish(fk_ctx *c, bool warm) {
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nchunk + 3) / 4, 1024));
   if (warm)
     LAUNCH_TIMED(c, "relax_list", hipLaunchKernelGGL(relax_list_k<true>, dim3(grid), dim3(256), 0, c->stream, c->p,
-                                                     c->U[c->ia], c->rlist, m.w, m.wb, norm_of(m), cnt, W));
+                                                     c->U[c->ia], c->rlist, m.w, m.wb, norm_of(m), cnt, W, c->nc,
+                                                     c->rfall));
   else
     LAUNCH_TIMED(c, "relax_list", hipLaunchKernelGGL(relax_list_k<false>, dim3(grid), dim3(256), 0, c->stream, c->p,
-                                                     c->U[c->ia], c->rlist, m.w, m.wb, norm_of(m), cnt, W));
+                                                     c->U[c->ia], c->rlist, m.w, m.wb, norm_of(m), cnt, W, c->nc,
+                                                     c->rfall));
+  const unsigned gfb = (unsigned)std::max<int64_t>(1, std::min<int64_t>((c->nc + 255) / 256, 256));
+  LAUNCH_TIMED(c, "relax_fallback", hipLaunchKernelGGL(relax_fallback_k, dim3(gfb), dim3(256), 0, c->stream, c->p,
+                                                       c->U[c->ia], c->rfall, cnt, W));
   c->w_valid = W != nullptr;
   return FK_OK;
 }

@@ -553,65 +553,97 @@ FK_HD inline int plain_cell(const fk_params &P, double *u, int *iters, int *tsat
   return chem;
 }
 
-// Chemical (p-T-g) equilibrium of chemRlx (chemRlx.cu:52-76, 195-215) by
-// Newton with a forward-difference Jacobian (step 1e-7 |x|) and full steps,
-// from the warm start (alpha_k, rho_k, e_k), to |dx_i| <= tol |x_i|.
-// Returns iterations or -1 (out of bounds, singular, not converged).
-FK_HD inline int chem_polish(const fk_params &P, double *U, double ak0, double rk0, double ek0, double tol,
-                                  int itmax) {
-  double x0[6];
-  x0[0] = clampd(P, U[0]);
-  x0[1] = 1 - x0[0];
-  ChemSys ch;
-  ch.rhoMix = x0[0] * (U[1] / x0[0]) + x0[1] * (U[2] / x0[1]);
-  const double u = U[3] / ch.rhoMix, v = U[4] / ch.rhoMix, w = U[5] / ch.rhoMix;
-  ch.rEint = U[8] - 0.5 * ch.rhoMix * (u * u + v * v + w * w);
-  ch.pm = (x0[1] > x0[0]) ? 1 : 0;
-  const int pm = ch.pm, k = 1 - pm;
-  double x[6] = {0, 0, 0, 0, 0, 0};
-  x[k] = ak0;
-  x[k + 2] = rk0;
-  x[k + 4] = ek0;
+// Chemical (p-T-g) equilibrium of chemRlx (chemRlx.cu:52-76, 195-215): the
+// same equations (equal phasic pressures, temperatures and Gibbs energies of
+// gibbsFunction, mass and internal energy of the cell conserved), solved in
+// the unknowns (ln p, T) common to both phases.  Each phase's density and
+// energy follow from (p, T) through the stiffened-gas EOS,
+//   rho_k = (p + pinf_k) / ((g_k - 1) Cv_k T),
+//   rho_k e_k = (p + g_k pinf_k) / (g_k - 1) + eta_k rho_k,
+// the volume fraction from the mass balance,
+//   alpha_L = (rho - rho_G) / (rho_L - rho_G),
+// and the residuals are the internal energy balance
+//   R1 = alpha_L rho_L e_L + alpha_G rho_G e_G - rho e
+// and the Gibbs difference R2 = g_L(p, T) - g_G(p, T), with the analytic
+// Jacobian
+//   dg_k/dT = Cp_k - s_k - Cp_k / ln 10,
+//   dg_k/dp = T Cv_k (g_k - 1) / ((p + pinf_k) ln 10) - T d(et_k)/dp,
+// (s_k the entropy of gibbsFunction, et_G = satConst0 log10 p + satConst1).
+// The reference's unknowns (alpha_k, rho_k, e_k) of the secondary phase pass
+// a relative error of its energy to the primary phase amplified by the mass
+// ratio (10^3 .. 10^5 for liquid in vapour), which makes vapour pressures
+// negative and its Newton steps leave the domain from all but very close
+// starts; in (ln p, T) both phases stay on their EOS.  Steps are limited to
+// |d ln p| <= 1 and |dT| <= 0.2 T.  Converged when |d ln p| <= tol and
+// |dT| <= tol T; the state is then committed as chemRlx commits it (the
+// secondary phase from (p, T), the primary phase from the mass and energy
+// closure, so both are conserved to rounding).  Returns the iterations, or
+// -1 (singular or non-finite step, T <= 0, no convergence within itmax,
+// alpha_L outside (0, 1) -- no two-phase equilibrium at this (rho, e) -- or a
+// phase outside its EOS domain); the caller then falls back to the
+// reference solve.
+FK_HD inline int chem_polish(const fk_params &P, double *U, double T0, double p0, double tol, int itmax) {
+  constexpr double kLn10 = 2.302585092994045684;
+  const double a0 = clampd(P, U[0]);
+  const double rho = a0 * (U[1] / a0) + (1 - a0) * (U[2] / (1 - a0));
+  const double u = U[3] / rho, v = U[4] / rho, w = U[5] / rho;
+  const double rE = U[8] - 0.5 * rho * (u * u + v * v + w * w);
+  if (!(T0 > 0) || !__builtin_isfinite(T0) || !__builtin_isfinite(p0)) return -1;
+  double lp = fkmath::log(fmax(p0, 1.0)), T = T0;
   for (int it = 1; it <= itmax; ++it) {
-    double r[3], rd[3][3], e[3];
-    ch.res(P, x, &r[0], &r[1], &r[2]);
-    const int idx[3] = {k, k + 2, k + 4};
-    for (int q = 0; q < 3; ++q) {
-      const int j = idx[q];
-      e[q] = 1e-7 * fabs(x[j]) + 1e-30;
-      const double keep = x[j];
-      x[j] = keep + e[q];
-      ch.res(P, x, &rd[0][q], &rd[1][q], &rd[2][q]);
-      x[j] = keep;
+    const double p = fkmath::exp(lp);
+    double r[2], drp[2], drT[2], E[2], dEp[2], dET[2], g[2], gp[2], gT[2];
+    for (int q = 0; q < 2; ++q) {
+      const double gm = P.gamma[q], pi = P.pInf[q], cv = P.Cp[q] / gm;
+      r[q] = (p + pi) / (cv * (gm - 1) * T);
+      drp[q] = r[q] / (p + pi);
+      drT[q] = -r[q] / T;
+      E[q] = (p + gm * pi) / (gm - 1) + P.eta[q] * r[q];
+      dEp[q] = 1 / (gm - 1) + P.eta[q] * drp[q];
+      dET[q] = P.eta[q] * drT[q];
+      const double et = q == 0 ? 0.0 : P.satConst[0] * fkmath::log10(p) + P.satConst[1];
+      const double s = cv * (gm * fkmath::log10(T) - (gm - 1) * fkmath::log10(p + pi)) + et;
+      g[q] = P.Cp[q] * T + P.eta[q] - T * s;
+      gT[q] = P.Cp[q] - s - P.Cp[q] / kLn10;
+      const double dsp = -cv * (gm - 1) / ((p + pi) * kLn10) + (q == 0 ? 0.0 : P.satConst[0] / (p * kLn10));
+      gp[q] = -T * dsp;
     }
-    double J[3][3];
-    for (int i = 0; i < 3; ++i)
-      for (int j = 0; j < 3; ++j) J[i][j] = (rd[i][j] - r[i]) / e[j];
-    const double det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-                       J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
+    const double D = r[0] - r[1];
+    const double al = (rho - r[1]) / D;
+    const double dap = (-drp[1] - al * (drp[0] - drp[1])) / D, daT = (-drT[1] - al * (drT[0] - drT[1])) / D;
+    const double dE = E[0] - E[1];
+    const double R1 = E[1] + al * dE - rE, R2 = g[0] - g[1];
+    const double J00 = p * (dEp[1] + dap * dE + al * (dEp[0] - dEp[1]));  // d R1 / d ln p
+    const double J01 = dET[1] + daT * dE + al * (dET[0] - dET[1]);        // d R1 / d T
+    const double J10 = p * (gp[0] - gp[1]);
+    const double J11 = gT[0] - gT[1];
+    const double det = J00 * J11 - J01 * J10;
     if (!(fabs(det) > 0) || !__builtin_isfinite(det)) return -1;
-    // Cramer's rule for J dx = r
-    double dx[3];
-    for (int c = 0; c < 3; ++c) {
-      double M[3][3];
-      for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) M[i][j] = j == c ? r[i] : J[i][j];
-      dx[c] = (M[0][0] * (M[1][1] * M[2][2] - M[1][2] * M[2][1]) - M[0][1] * (M[1][0] * M[2][2] - M[1][2] * M[2][0]) +
-               M[0][2] * (M[1][0] * M[2][1] - M[1][1] * M[2][0])) / det;
+    double dl = (J11 * R1 - J01 * R2) / det, dT = (J00 * R2 - J10 * R1) / det;
+    if (!__builtin_isfinite(dl) || !__builtin_isfinite(dT)) return -1;
+    const double lim = fmax(fabs(dl), fabs(dT) / (0.2 * T));
+    if (lim > 1) {
+      dl /= lim;
+      dT /= lim;
     }
-    bool small = true;
-    for (int q = 0; q < 3; ++q) {
-      const int j = idx[q];
-      x[j] -= dx[q];
-      if (!(fabs(dx[q]) <= tol * fabs(x[j]))) small = false;
-    }
-    // bounds of checkUpdate (chemRlx.cu:139-162)
-    if (!(x[k] > 0 && x[k] < 1 && x[k + 2] > 0 && x[k] * x[k + 2] <= ch.rhoMix) || !__builtin_isfinite(x[k + 4])) return -1;
-    if (small) {
+    lp -= dl;
+    T -= dT;
+    if (!(T > 0)) return -1;
+    if (fabs(dl) <= tol && fabs(dT) <= tol * T) {
+      const double pf = fkmath::exp(lp);
+      const double rl = (pf + P.pInf[0]) / ((P.Cp[0] / P.gamma[0]) * (P.gamma[0] - 1) * T);
+      const double rg = (pf + P.pInf[1]) / ((P.Cp[1] / P.gamma[1]) * (P.gamma[1] - 1) * T);
+      const double aL = (rho - rg) / (rl - rg);
+      if (!(aL > 0 && aL < 1)) return -1;
+      const int pm = (1 - a0 > a0) ? 1 : 0, k = 1 - pm;
+      double x[6];
+      x[k] = k == 0 ? aL : 1 - aL;
+      x[k + 2] = k == 0 ? rl : rg;
+      x[k + 4] = ((pf + P.gamma[k] * P.pInf[k]) / (P.gamma[k] - 1) + P.eta[k] * x[k + 2]) / x[k + 2];
       const double s1 = x[k], s2 = x[k] * x[k + 2], s3 = x[k] * x[k + 2] * x[k + 4];
       x[pm] = 1 - s1;
-      x[pm + 2] = (ch.rhoMix - s2) / x[pm];
-      x[pm + 4] = (ch.rEint - s3) / (x[pm] * x[pm + 2]);
+      x[pm + 2] = (rho - s2) / x[pm];
+      x[pm + 4] = (rE - s3) / (x[pm] * x[pm + 2]);
       if (!(x[pm + 2] > 0)) return -1;
       if (!eos_domain(P, 0, x[2], x[4]) || !eos_domain(P, 1, x[3], x[5])) return -1;
       U[0] = clampd(P, x[0]);
@@ -625,6 +657,44 @@ FK_HD inline int chem_polish(const fk_params &P, double *U, double ak0, double r
   }
   return -1;
 }
+
+// Cold start of the chemical polish: the liquid's own temperature and
+// pressure (at least 1 kPa).  Chemical cells are those whose liquid is
+// superheated, so (T_L, p_L) lies on the liquid side of the saturation curve
+// the solution sits on.
+FK_HD inline void chem_cold(const fk_params &P, const double *U, double &T0, double &p0) {
+  const double a = clampd(P, U[0]);
+  const double rl = U[1] / a, el = U[6] / U[1];
+  T0 = eos_T(P, 0, rl, el);
+  p0 = fmax(eos_p(P, 0, rl, el), 1e3);
+}
+
+#if defined(__HIPCC__)
+// Append of cell c to the surrogate relaxation's work list, wave-aggregated:
+// chemical cells from the front (their count in cnt[5]), thermal cells from
+// the back (count in cnt[6], k-th entry at list[cap - 1 - k]); entries are
+// (c << 1) | chem.  Stage 2 then runs the network on the waves of chemical
+// entries only.  The list order does not affect results: every cell is
+// solved independently.
+__device__ inline void list_push(bool push, int chem, int64_t c, uint32_t *__restrict__ list, int64_t cap,
+                                 unsigned long long *__restrict__ cnt) {
+  const int l = threadIdx.x & 63;
+#pragma unroll
+  for (int side = 0; side < 2; ++side) {
+    const bool mine = push && (side == 0) == (chem != 0);
+    const unsigned long long m = __ballot(mine);
+    if (!m) continue;
+    const int leader = __ffsll((long long)m) - 1;
+    unsigned long long b = 0;
+    if (l == leader) b = atomicAdd(&cnt[5 + side], (unsigned long long)__popcll(m));
+    b = __shfl(b, leader, 64);
+    if (mine) {
+      const int64_t k = (int64_t)(b + (unsigned long long)__popcll(m & ((1ull << l) - 1ull)));
+      list[side == 0 ? k : cap - 1 - k] = (uint32_t)((c << 1) | chem);
+    }
+  }
+}
+#endif
 
 // Surrogate features of a conserved state: (log rho, e, Y_L) with e the
 // mixture internal energy per unit mass and Y_L the liquid mass fraction.

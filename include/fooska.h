@@ -39,7 +39,8 @@ extern "C" {
 #define FK_RELAX_PLAIN 0 /* reference Newton solves (relaxation.cu, thermRlx.cu, chemRlx.cu) */
 #define FK_RELAX_MLP 1   /* MLP warm start + FP64 Newton polish, plain Newton as fallback */
 #define FK_RELAX_COLD 2  /* FK_RELAX_MLP with the cell state in place of the MLP
-                            start (ablation: what the network adds) */
+                            start, (T_L, max(p_L, 1 kPa)) for a chemical cell
+                            (ablation: what the network adds) */
 
 /* Solver and material constants; field names follow the reference INPUT
  * struct (strdata.h:49-98).  Two-phase: index 0 = liquid, 1 = gas.
@@ -210,9 +211,9 @@ int fk_set_debug(fk_ctx *ctx, int on);
  *     reduced, and a streaming kernel applies the update); at 48M cells,
  *     mixed: 7.5 ms against 6.7 + 2.5 ms.  0 = always residual + update
  *     kernel.
- *  "stage_f" (default 1): mixed precision only; pass F copies the face states
- *     and velocity gradients of its workgroup's cells to LDS and reads
- *     in-workgroup mirror states from there.
+ *  "stage_f" (default 1): meshes of tetrahedra; pass F copies the face
+ *     states and velocity gradients of its workgroup's cells (256, FP64 128)
+ *     to LDS and reads in-workgroup mirror states from there.
  *  "prims_w" (default 1): mixed precision only; the kernels that produce a
  *     new state (fused pass F, relaxation) also write its FP32 primitive
  *     variables, which the next pass G gathers instead of converting the FP64
@@ -233,8 +234,10 @@ int fk_set_tuning(fk_ctx *ctx, const char *name, int64_t value);
  * FK_RELAX_MLP classifies every cell exactly (sign of the saturation
  * function) and solves thermal cells by analytic-Jacobian Newton from the
  * cell state (at most 4 iterations in a first pass, up to 30 in a second);
- * chemical cells are solved by an FP64 Newton polish from the MLP warm start.
- * Where a polish fails the plain reference solve runs from the cell state.
+ * chemical cells are solved by an FP64 Newton polish in (ln p, T) from the
+ * MLP's (T, p) (the alpha_L output is not used: the polish takes alpha_L
+ * from the mass balance).  Where a polish fails the plain reference solve
+ * runs from the cell state.
  * The network is trained on chemical-class states only (fk_mlp_label and
  * the emission give NaN targets to other states and to states the plain
  * solve left out of equilibrium).

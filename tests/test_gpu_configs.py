@@ -23,8 +23,9 @@ oracle; mixed path (FP32 reconstruction, HLLC, WALE; FP64 state and Newton)
 with the momentum, energy and vapour-fraction errors printed per case.  The
 surrogate relaxation replaces the reference's Newton solves (FD Jacobian,
 absolute tolerances 1e-2 / 0.1) by a warm-started analytic polish to 1e-10
-relative, so it is held to the mixed-path density bar and to 1e-6 in every
-conserved variable in FP64."""
+relative, so it is held to the mixed-path density bar and, in FP64, to 1e-6
+in every conserved variable but the vapour partial density that only the
+chemical relaxation writes (1e-4: the reference's 0.1 K tolerance)."""
 import numpy as np
 import pytest
 
@@ -169,4 +170,10 @@ def test_surrogate_relaxation_matches_oracle_over_the_run(gpu, name, precision):
     assert np.isfinite(Ug).all()
     assert e["density"] <= MIXED_RHO_TOL
     if precision == _lib.FP64:
-        assert err.max() <= 1e-6
+        # variables 0-8 to 1e-6; the vapour partial density U[9], which only
+        # the chemical relaxation writes (chemRlx.cu updateConsVec), to 1e-4:
+        # the reference solve stops at |T_L - T_G| <= 0.1 K (chemRlx.cu
+        # convergence test), a relative error of up to ~3e-4 in alpha_G
+        # rho_G, and the polish converges to 1e-10, so a polished chemical
+        # cell differs from the reference's by the reference's own tolerance
+        assert err[:9].max() <= 1e-6 and err[9] <= 1e-4
