@@ -87,6 +87,8 @@ struct fk_ctx {
   // relaxation work list (chemical cells / surrogate list), one entry per cell
   uint32_t *rlist = nullptr;
   uint32_t *rfall = nullptr;   // surrogate relaxation: cells whose polish failed
+  void *rstart = nullptr;      // surrogate relaxation: network starts (float2) of the chemical entries
+  int64_t rstart_cap = 0;
   unsigned long long *rpart = nullptr;  // per-workgroup relaxation counts [blocks][4]
   int64_t rlist_cap = 0;
   double dt = 0, time = 0;
@@ -114,6 +116,7 @@ int fk_relax_mlp_impl(fk_ctx *c, bool warm);
 int fk_relax_mlp_begin(fk_ctx *c, bool warm);   // checks, scratch, zeroed counts
 int fk_relax_mlp_finish(fk_ctx *c, bool warm);  // stage 2 (work list) after stage 1
 int ensure_rlist(fk_ctx *c);  // defined in fk_kernels.hip
+int ensure_rstart(fk_ctx *c);  // defined in fk_kernels.hip
 int fk_relax_emit_impl(fk_ctx *c, bool *done);
 void fk_mlp_free(fk_ctx *c);
 

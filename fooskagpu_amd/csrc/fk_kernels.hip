@@ -982,7 +982,7 @@ int upload_geometry(fk_ctx *c) {
 void free_all(fk_ctx *c) {
   void *ptrs[] = {c->link, c->geo, c->rvec, c->wls, c->vol, c->volcfl, c->U[0], c->U[1], c->U[2], c->face,
                   c->gvel, c->rhs, c->dtc, c->red, c->dscal, c->dgrad, c->dtheta, c->lcount, c->rlist,
-                  c->rfall, c->rpart, c->wprim, c->health};
+                  c->rfall, c->rstart, c->rpart, c->wprim, c->health};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   c->link = nullptr;
@@ -991,6 +991,8 @@ void free_all(fk_ctx *c) {
   c->w_valid = false;
   c->rlist = nullptr;
   c->rfall = nullptr;
+  c->rstart = nullptr;
+  c->rstart_cap = 0;
   c->rpart = nullptr;
   c->rlist_cap = 0;
   c->geo = c->rvec = c->wls = nullptr;
@@ -1243,6 +1245,17 @@ int read_counts(fk_ctx *c, int base, fk_relax_stats *st) {
 }  // namespace
 
 // work list of the relaxation passes (one uint32 per cell), allocated on first use
+// Network starts (float2 per work-list entry) of the surrogate relaxation.
+int ensure_rstart(fk_ctx *c) {
+  if (c->rstart_cap >= c->nc) return FK_OK;
+  if (c->rstart) (void)hipFree(c->rstart);
+  c->rstart = nullptr;
+  c->rstart_cap = 0;
+  HIPCHK(hipMalloc(&c->rstart, (size_t)c->nc * 2 * sizeof(float)));
+  c->rstart_cap = c->nc;
+  return FK_OK;
+}
+
 int ensure_rlist(fk_ctx *c) {
   if (c->nc >= (int64_t(1) << 31)) return fail(FK_ERANGE, "relaxation supports < 2^31 cells");
   if (c->rlist_cap >= c->nc) return FK_OK;
@@ -1274,7 +1287,9 @@ int relax_preflight(fk_ctx *c, int mode) {
     if (P.chemFlag == 1 && !((P.eta[0] - P.eta[1]) <= 0 && (P.Cp[1] - P.Cp[0]) >= 0))
       return fail(FK_EINVAL, "surrogate classification needs eta_L <= eta_G and Cp_G >= Cp_L");
   }
-  return ensure_rlist(c);
+  int rc = ensure_rlist(c);
+  if (rc) return rc;
+  return mode == FK_RELAX_MLP ? ensure_rstart(c) : FK_OK;
 }
 
 int do_relax(fk_ctx *c, fk_relax_stats *st, int mode = FK_RELAX_PLAIN) {

@@ -455,32 +455,68 @@ __global__ void __launch_bounds__(256) relax_screen_k(fk_params P, int64_t nc, d
   fk::block_partials<2>(v, part);  // [n_chem, newton iterations]
 }
 
-// Surrogate relaxation, stage 2 (the work list, 64 entries per wave): MLP
-// warm start by wave-cooperative inference (WARM), or the cell state (the
-// ablation without the network), FP64 Newton polish, and the plain reference
-// solve from the cell state when the polish fails.
-template <bool WARM>
-__global__ void __launch_bounds__(256) relax_list_k(fk_params P, double *__restrict__ U,
-                                                    const uint32_t *__restrict__ list, const float *__restrict__ w,
-                                                    const uint16_t *__restrict__ wbg, Norm nm,
-                                                    unsigned long long *__restrict__ counts, float *__restrict__ W,
-                                                    int64_t cap, uint32_t *__restrict__ fall) {
+// Surrogate relaxation, stage 2a (FK_RELAX_MLP): the network's equilibrium
+// (T*, p*) of each chemical entry of the work list (its front),
+// by wave-cooperative inference, 64 entries per wave, into start[e].  Apart
+// from the polish (stage 2b) so that the registers of neither limit the
+// occupancy of the other.
+__global__ void __launch_bounds__(256) relax_infer_k(fk_params P, const double *__restrict__ U,
+                                                     const uint32_t *__restrict__ list, const float *__restrict__ w,
+                                                     const uint16_t *__restrict__ wbg, Norm nm,
+                                                     const unsigned long long *__restrict__ counts,
+                                                     float2 *__restrict__ start) {
   __shared__ __attribute__((aligned(16))) uint16_t wb[8192];
   __shared__ FwdLds lds[4];
   __shared__ SmallW sw;
+  const int64_t nfr = (int64_t)counts[5];
+  const int64_t cf = (nfr + 63) / 64;
+  if ((int64_t)blockIdx.x * 4 >= cf) return;  // whole block idle: skip the weight load
+  load_small(w, sw);
+  load_wb(wbg, wb, 8192);
+  const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
+  const int64_t per = (cf + 3) / 4 * 4;
+  for (int64_t ch = (int64_t)blockIdx.x * 4 + wv; ch < per; ch += (int64_t)gridDim.x * 4) {
+    const int64_t e = ch * 64 + l;
+    const bool live = e < nfr;
+    const int64_t c = live ? (int64_t)(list[e] >> 1) : 0;
+    double u[10];
+    for (int v = 0; v < 10; ++v) u[v] = live ? U[c * 10 + v] : (v == 1 || v == 2 ? 1.0 : 0.0);
+    float xf[3];
+    fk::mlp_features(P, u, xf);
+    float yw[3];
+    for (int t = 0; t < 4; ++t) {  // wave-uniform: every lane joins
+      if ((l >> 4) == t)
+        for (int j = 0; j < 3; ++j) lds[wv].x[(l & 15) * 3 + j] = (xf[j] - nm.xm[j]) / nm.xs[j];
+      __builtin_amdgcn_s_waitcnt(0);
+      __builtin_amdgcn_wave_barrier();
+      mlp_fwd16(sw, wb, lds[wv], nullptr, nullptr, nullptr);
+      if ((l >> 4) == t)
+        for (int j = 0; j < 3; ++j) yw[j] = lds[wv].y[(l & 15) * 3 + j] * nm.ys[j] + nm.ym[j];
+      __builtin_amdgcn_wave_barrier();
+    }
+    // (T*, p*); the alpha_L* output is not needed: the polish takes alpha_L
+    // from the mass balance at (p, T)
+    if (live) start[e] = make_float2(yw[0], (float)((double)kPScale * sinh((double)yw[1])));
+  }
+}
+
+// Surrogate relaxation, stage 2b (the work list, 64 entries per wave): FP64
+// Newton polish from the network's start (WARM, chemical entries) or from the
+// cell state (thermal entries, and FK_RELAX_COLD); a cell whose polish fails
+// goes to the fallback list.
+template <bool WARM>
+__global__ void __launch_bounds__(256) relax_list_k(fk_params P, double *__restrict__ U,
+                                                    const uint32_t *__restrict__ list,
+                                                    const float2 *__restrict__ start,
+                                                    unsigned long long *__restrict__ counts, float *__restrict__ W,
+                                                    int64_t cap, uint32_t *__restrict__ fall) {
   // chunks of 64 entries: the chemical ones from the list's front, then the
   // thermal ones from its back
   const int64_t nfr = (int64_t)counts[5], nbk = (int64_t)counts[6];
   const int64_t cf = (nfr + 63) / 64, nchunk = cf + (nbk + 63) / 64;
-  if ((int64_t)blockIdx.x * 4 >= nchunk) return;  // whole block idle: skip the weight load
-  if (WARM && nfr > 0) {
-    load_small(w, sw);
-    load_wb(wbg, wb, 8192);
-  }
   const int wv = threadIdx.x >> 6, l = threadIdx.x & 63;
   unsigned long long nit = 0, nwarm = 0;
-  const int64_t per = (nchunk + 3) / 4 * 4;
-  for (int64_t ch = (int64_t)blockIdx.x * 4 + wv; ch < per; ch += (int64_t)gridDim.x * 4) {
+  for (int64_t ch = (int64_t)blockIdx.x * 4 + wv; ch < nchunk; ch += (int64_t)gridDim.x * 4) {
     const bool front = ch < cf;  // wave-uniform
     const int64_t e = (front ? ch : ch - cf) * 64 + l;
     const bool live = e < (front ? nfr : nbk);
@@ -488,26 +524,12 @@ __global__ void __launch_bounds__(256) relax_list_k(fk_params P, double *__restr
     const int64_t c = (int64_t)(ent >> 1);
     const int chem = (int)(ent & 1u);
     double u[10];
-    for (int v = 0; v < 10; ++v) u[v] = live ? U[c * 10 + v] : (v == 0 ? 0.5 : (v == 1 || v == 2 ? 1.0 : 0.0));
+    for (int v = 0; v < 10; ++v) u[v] = live ? U[c * 10 + v] : 0.0;
     double T0 = 0, p0 = 0;  // chemical cells: start (T, p)
-    if (WARM && front) {  // wave-uniform: every lane joins the cooperative inference
-      float xf[3];
-      fk::mlp_features(P, u, xf);
-      float yw[3];
-      for (int t = 0; t < 4; ++t) {
-        if ((l >> 4) == t)
-          for (int j = 0; j < 3; ++j) lds[wv].x[(l & 15) * 3 + j] = (xf[j] - nm.xm[j]) / nm.xs[j];
-        __builtin_amdgcn_s_waitcnt(0);
-        __builtin_amdgcn_wave_barrier();
-        mlp_fwd16(sw, wb, lds[wv], nullptr, nullptr, nullptr);
-        if ((l >> 4) == t)
-          for (int j = 0; j < 3; ++j) yw[j] = lds[wv].y[(l & 15) * 3 + j] * nm.ys[j] + nm.ym[j];
-        __builtin_amdgcn_wave_barrier();
-      }
-      // predicted equilibrium (T*, p*); the alpha_L* output is not needed:
-      // the polish takes alpha_L from the mass balance at (p, T)
-      T0 = (double)yw[0];
-      p0 = (double)kPScale * sinh((double)yw[1]);
+    if (WARM && front && live) {
+      const float2 y = start[e];
+      T0 = (double)y.x;
+      p0 = (double)y.y;
     }
     if (!live) continue;
     if (chem && !WARM) fk::chem_cold(P, u, T0, p0);
@@ -721,14 +743,18 @@ int fk_relax_mlp_finish(fk_ctx *c, bool warm) {
   float *W = (c->p.precision == FK_MIXED && c->prims_w) ? c->wprim : nullptr;
   const int64_t nchunk = (c->nc + 63) / 64;
   const unsigned grid = (unsigned)std::max<int64_t>(1, std::min<int64_t>((nchunk + 3) / 4, 1024));
-  if (warm)
+  if (warm) {
+    int rc = ensure_rstart(c);  // fk_run_steps has allocated it before its first stage
+    if (rc) return rc;
+    float2 *st = static_cast<float2 *>(c->rstart);
+    LAUNCH_TIMED(c, "relax_infer", hipLaunchKernelGGL(relax_infer_k, dim3(grid), dim3(256), 0, c->stream, c->p,
+                                                      c->U[c->ia], c->rlist, m.w, m.wb, norm_of(m), cnt, st));
     LAUNCH_TIMED(c, "relax_list", hipLaunchKernelGGL(relax_list_k<true>, dim3(grid), dim3(256), 0, c->stream, c->p,
-                                                     c->U[c->ia], c->rlist, m.w, m.wb, norm_of(m), cnt, W, c->nc,
-                                                     c->rfall));
-  else
+                                                     c->U[c->ia], c->rlist, st, cnt, W, c->nc, c->rfall));
+  } else {
     LAUNCH_TIMED(c, "relax_list", hipLaunchKernelGGL(relax_list_k<false>, dim3(grid), dim3(256), 0, c->stream, c->p,
-                                                     c->U[c->ia], c->rlist, m.w, m.wb, norm_of(m), cnt, W, c->nc,
-                                                     c->rfall));
+                                                     c->U[c->ia], c->rlist, nullptr, cnt, W, c->nc, c->rfall));
+  }
   const unsigned gfb = (unsigned)std::max<int64_t>(1, std::min<int64_t>((c->nc + 255) / 256, 256));
   LAUNCH_TIMED(c, "relax_fallback", hipLaunchKernelGGL(relax_fallback_k, dim3(gfb), dim3(256), 0, c->stream, c->p,
                                                        c->U[c->ia], c->rfall, cnt, W));
