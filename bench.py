@@ -85,7 +85,7 @@ def measured_traffic(kernel, cells):
     return None if k is None else k["read_bytes_per_dispatch"] + k["write_bytes_per_dispatch"]
 
 
-def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None, fuse=1, stage_f=1):
+def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None, fuse=1, stage_f=1, stage_g=1):
     """Time `steps` explicitLoop steps on the device (state resident in HBM).
     train: dict of surrogate-training options (done before the timed region)."""
     from fooskagpu_amd import GpuSolver, surrogate
@@ -93,6 +93,7 @@ def run_gpu(case, precision, steps, warmup, relax_mode=0, device=0, train=None, 
     g = GpuSolver(p, case.mesh, case.U, device=device)
     g.set_tuning("fuse_update", fuse)
     g.set_tuning("stage_f", stage_f)
+    g.set_tuning("stage_g", stage_g)
     info = {}
     if train is not None:
         t0 = time.perf_counter()
@@ -238,6 +239,8 @@ def main():
                     help="pass F stages the workgroup's face states in LDS (mixed precision); same results")
     ap.add_argument("--fuse", type=int, default=1, choices=[0, 1],
                     help="RK update inside pass F (1) or residual + separate update kernel (0); same results")
+    ap.add_argument("--stage-g", type=int, default=1, choices=[0, 1],
+                    help="pass G writes its face records through LDS (mixed precision); same results")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30,
                     help="size of the plain device copy measured as the HBM reference (0: skip)")
     args = ap.parse_args()
@@ -271,7 +274,7 @@ def main():
             continue
         log(f"{name}: warmup {args.warmup}, timing {args.steps} steps")
         el, kt, U, dt, info = run_gpu(case, prec, args.steps, args.warmup, relax_mode=mode, train=tr, fuse=args.fuse,
-                                      stage_f=args.stage_f)
+                                      stage_f=args.stage_f, stage_g=args.stage_g)
         rate = C * s * args.steps / el
         log(f"{name}: {el:.3f} s, {rate:.4g} cell-stage/s, last dt {dt:.3e}, finite={bool(np.isfinite(U).all())}")
         for k, (n, ms) in sorted(kt.items(), key=lambda kv: -kv[1][1]):
@@ -316,7 +319,7 @@ def main():
                                   + ("present" if has_chem else "none in the timed steps") + ")"
                                   if head == "mixed_mlp" else "plain Newton relaxation"),
                    "cells": C, "faces": case.mesh.n_faces, "cell_order": args.order, "fuse_update": args.fuse,
-                   "stage_f": args.stage_f, "fuse_relax": 1},
+                   "stage_f": args.stage_f, "stage_g": args.stage_g, "fuse_relax": 1},
         "roofline": {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK, "traffic": traffic,
                      "kernel": dom, "bytes_per_cell": bpc, "avg_launch_ms": avg_f * 1e3},

@@ -99,6 +99,7 @@ struct fk_ctx {
   bool stage_f = true;       // pass F stages the workgroup's face states in LDS (tetrahedral meshes)
   bool prims_w = true;       // mixed: producers of U also write wprim, pass G reads it
   bool fuse_relax = true;    // per-cell relaxation pass inside the last fused pass F
+  bool stage_g = true;       // mixed pass G writes its face records through LDS (4 slots per cell)
   int64_t pf_waves = 0;      // waves of the last pass F launch (its per-wave relaxation counts)
   // FP32 primitive records [C][NEQ] of U[ia] (mixed precision), valid while
   // w_valid: set by the kernels that write them with a new state, cleared by

@@ -138,15 +138,36 @@ __device__ __forceinline__ int link_of(T w) {
 // come from the FP32 primitive records Wp written by the producer of U (the
 // same values, half the gathered bytes, no FP64 conversions here: 6.45
 // against 7.8 ms at 48M cells).
-template <int NEQ, typename T, bool VISC, bool HEAT, bool FROMW, int NS>
+// Floats per cell of pass G's LDS copy of its face records: the record
+// length rounded up to 16-byte words, and to an odd count of them, so that
+// the 64 lanes' rows start in different LDS banks (a 160-byte row stride
+// maps lanes 8 apart to one bank).
+template <int N>
+constexpr int gout_stride() {
+  constexpr int w = (N + 3) / 4;
+  return 4 * (w % 2 ? w : w + 1);
+}
+
+template <int NEQ, typename T, bool VISC, bool HEAT, bool FROMW, int NS, bool SOUT = false>
 __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
     pass_g(fk_params P, int64_t nc, const double *__restrict__ U, const float *__restrict__ Wp,
            const int32_t *__restrict__ link, const void *__restrict__ geo, const void *__restrict__ rvec,
            const double *__restrict__ vol, T *__restrict__ face,
            T *__restrict__ gvel, double *__restrict__ dgrad, double *__restrict__ dtheta,
            unsigned long long *__restrict__ health, int step) {
-  const int64_t c = xcd_block() * kBlock + threadIdx.x;
-  if (c >= nc) return;
+  static_assert(!SOUT || (sizeof(T) == 4 && NS * NEQ % 4 == 0), "face records staged as 16-byte words");
+  constexpr int SP = gout_stride<NS * NEQ>();  // floats per cell in the LDS copy of the face records
+  __shared__ __attribute__((aligned(16))) float sOut[SOUT ? kBlock * SP : 1];
+  const int64_t c0 = xcd_block() * kBlock;
+  const int64_t cl = c0 + threadIdx.x;
+  if constexpr (!SOUT) {
+    if (cl >= nc) return;
+  }
+  // SOUT: every thread reaches the block's barrier; those past the last cell
+  // compute the last cell again and write only to their own LDS row, which
+  // the copy-out does not read
+  const bool live = cl < nc;
+  const int64_t c = live ? cl : nc - 1;
   const fk::Mat<double> md = fk::make_mat<double>(P);
   const fk::Mat<T> mt = fk::make_mat<T>(P);
   T W[NEQ];
@@ -174,7 +195,7 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) sum += W[v];
     const bool ok = (NEQ == 10 ? (W[1] > 0 && W[2] > 0) : W[0] > 0) && fabs(sum) < (T)INFINITY;
-    if (!ok) atomicMin(health, ((unsigned long long)(unsigned)step << 32) | (unsigned long long)c);
+    if (!ok && live) atomicMin(health, ((unsigned long long)(unsigned)step << 32) | (unsigned long long)c);
   }
   T g[NEQ][3], mn[NEQ], mx[NEQ];
 #pragma unroll
@@ -282,12 +303,27 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
     // compiler from holding those 4 NEQ products in registers until here (it
     // recomputes them instead).  FP32 measured faster with them held.
     if constexpr (sizeof(T) == 8) asm volatile("" : "+v"(r.x), "+v"(r.y), "+v"(r.z));
-    T *out = face + (c * NS + j) * NEQ;
+    T *out = SOUT ? reinterpret_cast<T *>(sOut) + threadIdx.x * SP + j * NEQ : face + (c * NS + j) * NEQ;
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) {
       const T ext = r.x * g[v][0] + r.y * g[v][1] + r.z * g[v][2];
       out[v] = W[v] + th[v] * ext;
     }
+  }
+  if constexpr (SOUT) {
+    // the block's face records are one contiguous range: written from LDS in
+    // 16-byte words, each wave store covering 1 KB contiguously (a thread's
+    // own 160-byte record otherwise goes out as strided partial lines)
+    __syncthreads();
+    constexpr int WPC = NS * NEQ / 4;  // 16-byte words per cell
+    const int64_t ncell = (nc - c0) < kBlock ? (nc - c0) : kBlock;
+    const int nw = (int)ncell * WPC;
+    uint4 *dst = reinterpret_cast<uint4 *>(face + c0 * NS * NEQ);
+    for (int w = threadIdx.x; w < nw; w += kBlock) {
+      const int cell = w / WPC, part = w - cell * WPC;
+      dst[w] = *reinterpret_cast<const uint4 *>(sOut + cell * SP + part * 4);
+    }
+    if (!live) return;
   }
   if constexpr (VISC) {
     constexpr int iu = Model<NEQ>::iu;
@@ -1014,11 +1050,23 @@ int launch_pass_g_t(fk_ctx *c) {
   T *face = (T *)c->face;
   T *gv = (T *)c->gvel;
   const bool fromw = sizeof(T) == 4 && c->prims_w && c->w_valid;
-#define PG2(NEQ, VISC, HEAT, FW)                                                                                     \
+  // face records through LDS: mixed precision, 4 slots per cell
+  const bool sout = sizeof(T) == 4 && NS == 4 && c->stage_g;
+#define PG3(NEQ, VISC, HEAT, FW, SO)                                                                                  \
   LAUNCH_TIMED(c, "pass_g",                                                                                          \
-               hipLaunchKernelGGL((pass_g<NEQ, T, VISC, HEAT, FW, NS>), dim3(gr), dim3(kBlock), 0, c->stream, c->p,  \
-                                  c->nc, U, c->wprim, c->link, c->geo, c->rvec, c->vol, face, gv, dg, dth,  \
+               hipLaunchKernelGGL((pass_g<NEQ, T, VISC, HEAT, FW, NS, SO>), dim3(gr), dim3(kBlock), 0, c->stream,     \
+                                  c->p, c->nc, U, c->wprim, c->link, c->geo, c->rvec, c->vol, face, gv, dg, dth,    \
                                   c->health, (int)std::min<int64_t>(c->step, INT32_MAX)))
+#define PG2(NEQ, VISC, HEAT, FW)                                                                                     \
+  do {                                                                                                               \
+    if constexpr (sizeof(T) == 4 && NS == 4) {                                                                       \
+      if (sout) {                                                                                                    \
+        PG3(NEQ, VISC, HEAT, FW, true);                                                                              \
+        break;                                                                                                       \
+      }                                                                                                              \
+    }                                                                                                                \
+    PG3(NEQ, VISC, HEAT, FW, false);                                                                                 \
+  } while (0)
 #define PG(NEQ, VISC, HEAT)                     \
   do {                                          \
     if constexpr (sizeof(T) == 4) {             \
@@ -1039,6 +1087,7 @@ int launch_pass_g_t(fk_ctx *c) {
   }
 #undef PG
 #undef PG2
+#undef PG3
   return FK_OK;
 }
 
@@ -1699,6 +1748,10 @@ int fk_set_tuning(fk_ctx *c, const char *name, int64_t value) {
   }
   if (std::strcmp(name, "fuse_relax") == 0) {
     c->fuse_relax = value != 0;
+    return FK_OK;
+  }
+  if (std::strcmp(name, "stage_g") == 0) {
+    c->stage_g = value != 0;
     return FK_OK;
   }
   return fail(FK_EINVAL, std::string("unknown tuning option ") + name);

@@ -231,22 +231,45 @@ def test_cons_calc_matches_oracle(gpu):
 @pytest.mark.parametrize("precision", [_lib.FP64, _lib.MIXED])
 def test_tuning_options_do_not_change_results(gpu, precision, name):
     """fuse_update (RK update inside pass F or residual + separate update),
-    stage_f (LDS staging of face states in pass F) and prims_w (pass G reads
-    FP32 primitive records written by the producers of U) are implementation
-    choices: bitwise equal states after several steps (two-phase with
-    relaxation, single-phase viscous, single-phase inviscid)."""
+    stage_f (LDS staging of face states in pass F), prims_w (pass G reads
+    FP32 primitive records written by the producers of U) and stage_g (pass G
+    writes its face records through LDS) are implementation choices: bitwise
+    equal states after several steps (two-phase with relaxation,
+    single-phase viscous, single-phase inviscid)."""
     c = SMALL[name]()
     p = dict(c.params, precision=precision)
     out = []
-    for fuse, stage_f, prims_w in ((0, 0, 0), (1, 0, 1), (0, 1, 1), (1, 1, 0), (1, 1, 1)):
+    for fuse, stage_f, prims_w, stage_g in ((0, 0, 0, 0), (1, 0, 1, 1), (0, 1, 1, 0), (1, 1, 0, 1), (1, 1, 1, 1),
+                                            (1, 1, 1, 0)):
         with GpuSolver(p, c.mesh, c.U) as g:
             g.set_tuning("fuse_update", fuse)
             g.set_tuning("stage_f", stage_f)
             g.set_tuning("prims_w", prims_w)
+            g.set_tuning("stage_g", stage_g)
             g.run_steps(3)
             out.append(g.download_state())
     for o in out[1:]:
         assert np.array_equal(o, out[0])
+
+
+@pytest.mark.parametrize("n", [10, 17])
+def test_pass_g_face_records_through_lds_are_bitwise(gpu, n):
+    """stage_g: mixed pass G writes the face records of its workgroup from LDS
+    in 16-byte words; face states, gradients and limiter equal the direct
+    stores bitwise (6000 and 29478 cells: the last workgroup is partial)."""
+    c = cases.shock_bubble(n=n, nbub=3)
+    p = dict(c.params, precision=_lib.MIXED)
+    out = []
+    for sg in (0, 1):
+        with GpuSolver(p, c.mesh, c.U) as g:
+            g.set_tuning("stage_g", sg)
+            g.run_steps(1)
+            g.set_debug(True)
+            g.compute_face_values()
+            out.append([g.download(k) for k in ("face", "grad", "theta")])
+    assert c.mesh.n_cells % 256 != 0
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b, equal_nan=True)
 
 
 @pytest.mark.parametrize("name,mode", [("cavitation", _lib.RELAX_PLAIN), ("cavitation", _lib.RELAX_COLD),
