@@ -323,12 +323,14 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
       const int cell = w / WPC, part = w - cell * WPC;
       dst[w] = *reinterpret_cast<const uint4 *>(sOut + cell * SP + part * 4);
     }
-    if (!live) return;
   }
   if constexpr (VISC) {
     constexpr int iu = Model<NEQ>::iu;
     constexpr int NG = ngrad<NEQ, HEAT>();
-    T *G = gvel + c * NG;
+    // SOUT: the gradient rows go through the same LDS (after the face copy
+    // has read it), stride NG, and out as the block's contiguous range
+    if constexpr (SOUT) __syncthreads();
+    T *G = SOUT ? reinterpret_cast<T *>(sOut) + threadIdx.x * NG : gvel + c * NG;
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
@@ -341,7 +343,15 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
 #pragma unroll
       for (int d = 0; d < 3; ++d) G[9 + d] = (g[4][d] - ((W[4] + pi) / W[0]) * g[0][d]) / (W[0] * R);
     }
+    if constexpr (SOUT) {
+      __syncthreads();
+      const int64_t ncell = (nc - c0) < kBlock ? (nc - c0) : kBlock;
+      const int nf = (int)ncell * NG;
+      T *dg = gvel + c0 * NG;
+      for (int f = threadIdx.x; f < nf; f += kBlock) dg[f] = reinterpret_cast<const T *>(sOut)[f];
+    }
   }
+  if (SOUT && !live) return;
   if (dgrad) {
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) {
