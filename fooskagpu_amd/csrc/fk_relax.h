@@ -410,11 +410,13 @@ FK_HD inline void therm_cold(const fk_params &P, const double *U, const ThermPro
 }
 
 // A phase state inside the stiffened gas's domain: T > 0 and p + pinf > 0
-// (a real sound speed).  The equilibrium equations also have roots outside
-// it; the surrogate's solves reject those and leave the cell to the
-// reference solve.
+// (a real sound speed).  With T = (p + pinf) / (Cv (g - 1) rho) the two hold
+// together exactly when rho > 0 and p + pinf > 0, which is tested (no
+// division).  The equilibrium equations also have roots outside the domain;
+// the surrogate's solves reject those and leave the cell to the reference
+// solve.
 FK_HD inline bool eos_domain(const fk_params &P, int k, double rho, double e) {
-  return eos_T(P, k, rho, e) > 0 && eos_p(P, k, rho, e) + P.pInf[k] > 0;
+  return rho > 0 && eos_p(P, k, rho, e) + P.pInf[k] > 0;
 }
 
 // Newton iterations with the analytic Jacobian on (rho_k, e_k) from (xr, xe)
@@ -464,7 +466,7 @@ FK_HD inline int therm_newton(const fk_params &P, const ThermProb &t, double &xr
     xe -= dee;
     if (!(xr > t.ar[k]) || !__builtin_isfinite(xe)) return -1;
     if (fabs(dr) <= tol * fabs(xr) && fabs(dee) <= tol * fabs(xe)) {
-      const double rmf = t.ar[pm] / (1 - ark / xr), emf = em0 - ratio * (xe - ek0);
+      const double rmf = t.ar[pm] * xr / (xr - ark), emf = em0 - ratio * (xe - ek0);  // rho_m, e_m
       return (eos_domain(P, k, xr, xe) && eos_domain(P, pm, rmf, emf)) ? it : -1;
     }
   }
