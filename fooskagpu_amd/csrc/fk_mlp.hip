@@ -4,7 +4,8 @@
 // the reference (its relaxation.cu / thermRlx.cu / chemRlx.cu always start
 // from the cell state).
 //
-// Numerics of the network (the CPU check restates the same):
+// Numerics of the network (the CPU check restates the same; inference takes
+// tanh as tanh_fast, within ~1e-7 of it, training tanhf):
 //   layer 1: FP32 FMA, tanh, rounded to BF16
 //   layers 2, 3: BF16 x BF16 MFMA (v_mfma_f32_16x16x32_bf16), FP32
 //                accumulate, + bias, tanh, rounded to BF16
@@ -57,6 +58,14 @@ __device__ __forceinline__ uint16_t f2bf(float f) {  // round to nearest even
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float(((uint32_t)b) << 16); }
 
+// tanh(z) = 1 - 2 / (1 + e^(2z)) with the hardware exponential and
+// reciprocal: within about 1e-7 of tanh, far below the BF16 rounding
+// (2^-9 relative) of every activation.  Inference only (the relaxation's
+// warm start); training and its CPU check use tanhf.
+__device__ __forceinline__ float tanh_fast(float z) {
+  return 1.f - 2.f * __builtin_amdgcn_rcpf(1.f + __expf(2.f * z));
+}
+
 __device__ __forceinline__ bf16x8 ld_bf8(const uint16_t *p) {  // 16-byte aligned
   return *reinterpret_cast<const bf16x8 *>(p);
 }
@@ -81,6 +90,7 @@ struct SmallW {
 // lds.y [16][3] (normalised).  Weights: sw (LDS: W1, b1, b2, b3, W4, b4),
 // bf16 wb in LDS (W2, W3).  Keeps the BF16 activations of each layer in hs
 // (row-major [16][64]) when hs != nullptr (training).
+template <bool FAST = false>
 __device__ __forceinline__ void mlp_fwd16(const SmallW &sw, const uint16_t *wb, FwdLds &lds, uint16_t *h1s,
                                          uint16_t *h2s, uint16_t *h3s) {
   const int l = threadIdx.x & 63;
@@ -95,7 +105,7 @@ __device__ __forceinline__ void mlp_fwd16(const SmallW &sw, const uint16_t *wb, 
       const int k = 32 * ks + 8 * q + i;
       const float4 c = sw.w1b[k];
       const float z = c.w + c.x * x0 + c.y * x1 + c.z * x2;
-      const uint16_t hb = f2bf(tanhf(z));
+      const uint16_t hb = f2bf(FAST ? tanh_fast(z) : tanhf(z));
       a[ks][i] = __builtin_bit_cast(__bf16, hb);
       if (h1s) h1s[m * HID + k] = hb;
     }
@@ -117,7 +127,7 @@ __device__ __forceinline__ void mlp_fwd16(const SmallW &sw, const uint16_t *wb, 
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const int row = 4 * q + i;
-        const uint16_t hb = f2bf(tanhf(acc[i] + bias[n]));
+        const uint16_t hb = f2bf(FAST ? tanh_fast(acc[i] + bias[n]) : tanhf(acc[i] + bias[n]));
         lds.h[row * HID + n] = hb;
         if (hs) hs[row * HID + n] = hb;
       }
@@ -198,7 +208,7 @@ __global__ void __launch_bounds__(256) mlp_infer_k(int64_t n, const float *__res
       }
       __builtin_amdgcn_s_waitcnt(0);
       __builtin_amdgcn_wave_barrier();
-      mlp_fwd16(sw, wb, lds[wv], nullptr, nullptr, nullptr);
+      mlp_fwd16<true>(sw, wb, lds[wv], nullptr, nullptr, nullptr);
       if (l < 48) {
         const int mm = l / 3, j = l % 3;
         const int64_t s = base + mm;
@@ -489,7 +499,7 @@ __global__ void __launch_bounds__(256) relax_infer_k(fk_params P, const double *
         for (int j = 0; j < 3; ++j) lds[wv].x[(l & 15) * 3 + j] = (xf[j] - nm.xm[j]) / nm.xs[j];
       __builtin_amdgcn_s_waitcnt(0);
       __builtin_amdgcn_wave_barrier();
-      mlp_fwd16(sw, wb, lds[wv], nullptr, nullptr, nullptr);
+      mlp_fwd16<true>(sw, wb, lds[wv], nullptr, nullptr, nullptr);
       if ((l >> 4) == t)
         for (int j = 0; j < 3; ++j) yw[j] = lds[wv].y[(l & 15) * 3 + j] * nm.ys[j] + nm.ym[j];
       __builtin_amdgcn_wave_barrier();
