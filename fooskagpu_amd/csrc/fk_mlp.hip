@@ -404,8 +404,23 @@ __global__ void __launch_bounds__(256) mlp_pack_k(const float *__restrict__ w, u
   wb[bW3T + c * HID + r] = b;
 }
 
-// Labels host-provided states: features of U, then the plain relaxation and
-// the targets of the relaxed state.
+// Training target of a state: the chemical (p-T-g) equilibrium of its
+// mixture density and internal energy, (T, asinh(p / 1e4), alpha_L), by the
+// (ln p, T) Newton from the liquid's (T, p) to 1e-10 (fk::chem_polish), or NaN
+// where that solve finds no two-phase equilibrium.  The equilibrium depends
+// on (rho, e) alone, so the labels of every state, whatever its class, are
+// values of the one map the network learns.  u is not modified.
+__device__ inline void chem_label(const fk_params &P, const double *u, float *y) {
+  double uc[10];
+  for (int v = 0; v < 10; ++v) uc[v] = u[v];
+  double T0, p0;
+  fk::chem_cold(P, uc, T0, p0);
+  if (fk::chem_polish(P, uc, T0, p0, P.polishTol, 30) > 0 && fk::mlp_targets(P, uc, y)) return;
+  for (int j = 0; j < 3; ++j) y[j] = NAN;
+}
+
+// Labels host-provided states: features of U, and for the chemical-class
+// states (those the relaxation hands to the network) the equilibrium target.
 __global__ void __launch_bounds__(256) mlp_label_k(fk_params P, int64_t n, const double *__restrict__ U,
                                                    float *__restrict__ x, float *__restrict__ y) {
   const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -421,10 +436,8 @@ __global__ void __launch_bounds__(256) mlp_label_k(fk_params P, int64_t n, const
     const double tL = fk::eos_T(P, 0, rho, enr), pL = fk::eos_p(P, 0, rho, enr);
     chem = tL > fk::tsat_dev(P, pL, tL, P.satTempUrf, &ts) && avf > P.chemLimit;
   }
-  if (chem) fk::chem_dev(P, u);
-  else fk::therm_dev(P, u);
-  fk::mlp_targets(P, u, y + c * 3);
-  if (!chem)  // the network serves chemical cells only
+  if (chem) chem_label(P, u, y + c * 3);
+  else  // the network serves chemical cells only
     for (int j = 0; j < 3; ++j) y[c * 3 + j] = NAN;
 }
 
@@ -620,7 +633,8 @@ __global__ void __launch_bounds__(256) relax_fallback_k(fk_params P, double *__r
 }
 
 // Emission of solver states: plain relaxation that also writes (features,
-// targets) of cells c = phase + k stride into slot k of the training set.
+// equilibrium target) of the pre-relaxation states of cells c = phase +
+// k stride, whatever their class, into slot k of the training set.
 __global__ void __launch_bounds__(256) relax_emit_k(fk_params P, int64_t nc, double *__restrict__ U, int64_t stride,
                                                     int64_t phase, float *__restrict__ dx, float *__restrict__ dy,
                                                     int64_t cap) {
@@ -629,8 +643,11 @@ __global__ void __launch_bounds__(256) relax_emit_k(fk_params P, int64_t nc, dou
   double u[10];
   for (int v = 0; v < 10; ++v) u[v] = U[c * 10 + v];
   const bool emit = stride > 0 && (c % stride) == phase;
-  float xf[3];
-  if (emit) fk::mlp_features(P, u, xf);
+  float xf[3], yl[3];
+  if (emit) {
+    fk::mlp_features(P, u, xf);
+    chem_label(P, u, yl);  // from the pre-relaxation state, whatever the cell's class
+  }
   int chem = 0, ts = 0;
   if (P.chemFlag == 1) {
     const double avf = fk::clampd(P, u[0]);
@@ -644,10 +661,10 @@ __global__ void __launch_bounds__(256) relax_emit_k(fk_params P, int64_t nc, dou
   if (emit) {
     const int64_t slot = (c - phase) / stride;
     if (slot < cap) {
-      for (int j = 0; j < 3; ++j) dx[slot * 3 + j] = xf[j];
-      fk::mlp_targets(P, u, dy + slot * 3);
-      if (!chem)  // the network serves chemical cells only
-        for (int j = 0; j < 3; ++j) dy[slot * 3 + j] = NAN;
+      for (int j = 0; j < 3; ++j) {
+        dx[slot * 3 + j] = xf[j];
+        dy[slot * 3 + j] = yl[j];
+      }
     }
   }
 }

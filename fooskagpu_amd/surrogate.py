@@ -183,7 +183,7 @@ def train_on_solver(solver, case_U, params, n_solver=4_000_000, n_uniform=1_000_
     xs, ys = sg.data()
     xu, yu = sg.label(uniform_states(n_uniform, params, seed=seed + 1))
     log(f"surrogate: {len(xs)} solver states (stride {stride}) + {len(xu)} uniform-density states, "
-        f"{int(np.isfinite(ys).all(1).sum() + np.isfinite(yu).all(1).sum())} chemical-class equilibrium labels")
+        f"{int(np.isfinite(ys).all(1).sum())} + {int(np.isfinite(yu).all(1).sum())} equilibrium labels")
     x, y = finite(np.concatenate([xs, xu]), np.concatenate([ys, yu]))
     norm = fit_norm(x, y)
     sg.set_data(x, y)

@@ -240,9 +240,11 @@ int fk_set_tuning(fk_ctx *ctx, const char *name, int64_t value);
  * MLP's (T, p) (the alpha_L output is not used: the polish takes alpha_L
  * from the mass balance).  Where a polish fails the plain reference solve
  * runs from the cell state.
- * The network is trained on chemical-class states only (fk_mlp_label and
- * the emission give NaN targets to other states and to states the plain
- * solve left out of equilibrium).
+ * Training targets are chemical (p-T-g) equilibria of the states' mixture
+ * density and internal energy, solved to 1e-10 by the same (ln p, T) Newton
+ * from the cell state; NaN where no two-phase equilibrium exists.  The
+ * emission labels every emitted solver state; fk_mlp_label labels the
+ * chemical-class states it is given (NaN for the others).
  * Parameters are
  * FK_MLP_PARAMS floats in the order W1[64][3] b1 W2[64][64] b2 W3[64][64] b3
  * W4[3][64] b4; norm[12] = feature mean[3], std[3], target mean[3], std[3]. */
@@ -250,8 +252,9 @@ int fk_set_tuning(fk_ctx *ctx, const char *name, int64_t value);
 int fk_mlp_set_weights(fk_ctx *ctx, const float *w, const float *norm);
 int fk_mlp_get_weights(fk_ctx *ctx, float *w, float *norm);
 int fk_mlp_set_norm(fk_ctx *ctx, const float *norm);
-/* Features and plain-relaxation targets of host states U[n][10] (the
- * solver state is untouched). */
+/* Features and equilibrium targets of host states U[n][10] (targets NaN
+ * for thermal-class states and where no equilibrium exists; the solver
+ * state is untouched). */
 int fk_mlp_label(fk_ctx *ctx, const double *U, int64_t n, float *x, float *y);
 /* Training set on the device: replace (append = 0) or extend it. */
 int fk_mlp_data(fk_ctx *ctx, const float *x, const float *y, int64_t n, int append);

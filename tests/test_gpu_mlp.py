@@ -191,8 +191,8 @@ def test_surrogate_classification_check_fails_before_any_stage(gpu):
 
 def test_network_halves_the_newton_work_on_chemical_cells(gpu):
     """What the network adds where it is used: on chemical-class states
-    whose equilibrium exists (the plain reference solve reached p-T-g
-    equilibrium from them; the labelling keeps exactly those), the polish
+    whose equilibrium exists (those the labelling gives a finite target: the
+    (ln p, T) solve from the cell state reached the p-T-g equilibrium), the polish
     from the MLP prediction needs at most 0.6x the Newton iterations of the
     same polish from the cell state (FK_RELAX_COLD), falls back to the plain
     solve for fewer than 5 % of the cells, and leaves at least 95 % of them
