@@ -795,8 +795,17 @@ __global__ void __launch_bounds__(kBlock) rk_update_w(fk_params P, int64_t nc, c
                                                       const RT *__restrict__ rhs, const double *__restrict__ vol,
                                                       int irk, double dt_val, const double *__restrict__ dt_ptr,
                                                       float *__restrict__ W) {
-  const int64_t c = (int64_t)blockIdx.x * kBlock + threadIdx.x;
-  if (c >= nc) return;
+  // The new state and its primitive record leave through LDS: a thread's
+  // 80-byte (and 40-byte) record stored directly goes out as strided partial
+  // lines; from LDS the workgroup stores its contiguous ranges in 16-byte
+  // (state) and 8-byte (record) words.  Threads past the last cell compute
+  // the last cell again and write only their own LDS rows.
+  constexpr int SU = NEQ % 2 ? NEQ : NEQ + 1;  // doubles per LDS row: an odd count of 8-byte words
+  constexpr int SWR = NEQ % 2 ? NEQ : NEQ + 1; // floats per record row: an odd count
+  __shared__ __attribute__((aligned(16))) double sU[kBlock * SU];
+  __shared__ __attribute__((aligned(16))) float sWr[kBlock * SWR];
+  const int64_t c0 = (int64_t)blockIdx.x * kBlock;
+  const int64_t c = c0 + threadIdx.x < nc ? c0 + threadIdx.x : nc - 1;
   const double dt = dt_ptr ? *dt_ptr : dt_val;
   const double Vc = vol[c];
   constexpr int iz = NEQ == 5 ? 3 : 5;
@@ -819,8 +828,22 @@ __global__ void __launch_bounds__(kBlock) rk_update_w(fk_params P, int64_t nc, c
     else u = (1 - neglect) * ((1.0 / 3.0) * u0[v] + (2.0 / 3.0) * uc[v] + (2.0 / 3.0) * dt * (1.0 / Vc) * r[v]);
     un[v] = u;
   }
-  fk::store_rec<NEQ>(Unew + c * NEQ, un);
-  fk::store_prims<NEQ>(fk::make_mat<double>(P), un, W + c * NEQ);
+#pragma unroll
+  for (int v = 0; v < NEQ; ++v) sU[threadIdx.x * SU + v] = un[v];
+  fk::store_prims<NEQ>(fk::make_mat<double>(P), un, sWr + threadIdx.x * SWR);
+  __syncthreads();
+  const int ncell = (int)((nc - c0) < kBlock ? (nc - c0) : kBlock);
+  const int nu = ncell * NEQ;
+  double *du = Unew + c0 * NEQ;
+  for (int w = threadIdx.x; w < nu; w += kBlock) {
+    const int cell = w / NEQ;
+    du[w] = sU[cell * SU + (w - cell * NEQ)];
+  }
+  float *dw = W + c0 * NEQ;
+  for (int w = threadIdx.x; w < nu; w += kBlock) {
+    const int cell = w / NEQ;
+    dw[w] = sWr[cell * SWR + (w - cell * NEQ)];
+  }
 }
 
 // min over cells of Vcfl/signal: per-block minima, then one block folds them
