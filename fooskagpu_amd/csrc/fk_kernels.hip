@@ -140,8 +140,8 @@ __device__ __forceinline__ int link_of(T w) {
 // against 7.8 ms at 48M cells).
 // Floats per cell of pass G's LDS copy of its face records: the record
 // length rounded up to 16-byte words, and to an odd count of them, so that
-// the 64 lanes' rows start in different LDS banks (a 160-byte row stride
-// maps lanes 8 apart to one bank).
+// the lanes' rows start in different LDS banks.  (Measured at 48M cells:
+// rows of 40, 44 and 52 floats run within 0.5 % of each other.)
 template <int N>
 constexpr int gout_stride() {
   constexpr int w = (N + 3) / 4;
