@@ -139,13 +139,14 @@ __device__ __forceinline__ int link_of(T w) {
 // same values, half the gathered bytes, no FP64 conversions here: 6.45
 // against 7.8 ms at 48M cells).
 // Floats per cell of pass G's LDS copy of its face records: the record
-// length rounded up to 16-byte words, and to an odd count of them, so that
-// the lanes' rows start in different LDS banks.  (Measured at 48M cells:
-// rows of 40, 44 and 52 floats run within 0.5 % of each other.)
+// length rounded up to 16-byte words, no padding.  Rows of 40, 44 and 52
+// floats measured the same at 48M cells (bank conflicts of the row writes
+// do not show), and 40 (40 KB per workgroup) lets four workgroups share a
+// CU's LDS.
 template <int N>
 constexpr int gout_stride() {
   constexpr int w = (N + 3) / 4;
-  return 4 * (w % 2 ? w : w + 1);
+  return 4 * w;
 }
 
 template <int NEQ, typename T, bool VISC, bool HEAT, bool FROMW, int NS, bool SOUT = false>
@@ -155,10 +156,16 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
            const double *__restrict__ vol, T *__restrict__ face,
            T *__restrict__ gvel, double *__restrict__ dgrad, double *__restrict__ dtheta,
            unsigned long long *__restrict__ health, int step) {
-  static_assert(!SOUT || (sizeof(T) == 4 && NS * NEQ % 4 == 0), "face records staged as 16-byte words");
-  constexpr int SP = gout_stride<NS * NEQ>();  // floats per cell in the LDS copy of the face records
-  __shared__ __attribute__((aligned(16))) float sOut[SOUT ? kBlock * SP : 1];
-  const int64_t c0 = xcd_block() * kBlock;
+  static_assert(!SOUT || NS * NEQ * sizeof(T) % 16 == 0, "face records staged as 16-byte words");
+  // SOUT workgroup: 256 cells, FP64 128 (its 320-byte records fill 40 KB,
+  // so that four workgroups, the FP64 occupancy of 2 waves per SIMD, fit
+  // in a CU's LDS)
+  constexpr int GB = (SOUT && sizeof(T) == 8) ? 128 : kBlock;
+  // T per cell in the LDS copy of the face records (FP64: unpadded), and floats
+  constexpr int SPT = sizeof(T) == 4 ? gout_stride<NS * NEQ>() : NS * NEQ;
+  constexpr int SP = SPT * (int)sizeof(T) / 4;
+  __shared__ __attribute__((aligned(16))) float sOut[SOUT ? GB * SP : 1];
+  const int64_t c0 = xcd_block() * GB;
   const int64_t cl = c0 + threadIdx.x;
   if constexpr (!SOUT) {
     if (cl >= nc) return;
@@ -299,11 +306,13 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
 #pragma unroll
   for (int j = 0; j < NS; ++j) {
     V4<T> r = ld4<T>(rvec, c * NS + j);
-    // FP64: r g is also formed by the limiter loop; the empty asm keeps the
+    // r g is also formed by the limiter loop; the empty asm keeps the
     // compiler from holding those 4 NEQ products in registers until here (it
-    // recomputes them instead).  FP32 measured faster with them held.
-    if constexpr (sizeof(T) == 8) asm volatile("" : "+v"(r.x), "+v"(r.y), "+v"(r.z));
-    T *out = SOUT ? reinterpret_cast<T *>(sOut) + threadIdx.x * SP + j * NEQ : face + (c * NS + j) * NEQ;
+    // recomputes them instead): mixed 127 instead of 166 VGPRs, 4 waves per
+    // SIMD, 5.34 against 5.50 ms with the LDS store path (without it, when
+    // the stores bounded the kernel, holding them had measured faster).
+    asm volatile("" : "+v"(r.x), "+v"(r.y), "+v"(r.z));
+    T *out = SOUT ? reinterpret_cast<T *>(sOut) + threadIdx.x * SPT + j * NEQ : face + (c * NS + j) * NEQ;
 #pragma unroll
     for (int v = 0; v < NEQ; ++v) {
       const T ext = r.x * g[v][0] + r.y * g[v][1] + r.z * g[v][2];
@@ -315,11 +324,11 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
     // 16-byte words, each wave store covering 1 KB contiguously (a thread's
     // own 160-byte record otherwise goes out as strided partial lines)
     __syncthreads();
-    constexpr int WPC = NS * NEQ / 4;  // 16-byte words per cell
-    const int64_t ncell = (nc - c0) < kBlock ? (nc - c0) : kBlock;
+    constexpr int WPC = NS * NEQ * (int)sizeof(T) / 16;  // 16-byte words per cell
+    const int64_t ncell = (nc - c0) < GB ? (nc - c0) : GB;
     const int nw = (int)ncell * WPC;
     uint4 *dst = reinterpret_cast<uint4 *>(face + c0 * NS * NEQ);
-    for (int w = threadIdx.x; w < nw; w += kBlock) {
+    for (int w = threadIdx.x; w < nw; w += GB) {
       const int cell = w / WPC, part = w - cell * WPC;
       dst[w] = *reinterpret_cast<const uint4 *>(sOut + cell * SP + part * 4);
     }
@@ -345,10 +354,10 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
     }
     if constexpr (SOUT) {
       __syncthreads();
-      const int64_t ncell = (nc - c0) < kBlock ? (nc - c0) : kBlock;
+      const int64_t ncell = (nc - c0) < GB ? (nc - c0) : GB;
       const int nf = (int)ncell * NG;
       T *dg = gvel + c0 * NG;
-      for (int f = threadIdx.x; f < nf; f += kBlock) dg[f] = reinterpret_cast<const T *>(sOut)[f];
+      for (int f = threadIdx.x; f < nf; f += GB) dg[f] = reinterpret_cast<const T *>(sOut)[f];
     }
   }
   if (SOUT && !live) return;
@@ -1083,16 +1092,20 @@ int launch_pass_g_t(fk_ctx *c) {
   T *face = (T *)c->face;
   T *gv = (T *)c->gvel;
   const bool fromw = sizeof(T) == 4 && c->prims_w && c->w_valid;
-  // face records through LDS: mixed precision, 4 slots per cell
-  const bool sout = sizeof(T) == 4 && NS == 4 && c->stage_g;
+  // face records through LDS (4 slots per cell): workgroups of 256 cells,
+  // FP64 128
+  const bool sout = NS == 4 && c->stage_g;
+  constexpr int gbs = sizeof(T) == 8 ? 128 : kBlock;
+  const unsigned grs = (unsigned)(((c->nc + gbs - 1) / gbs + kXcd - 1) / kXcd * kXcd);
 #define PG3(NEQ, VISC, HEAT, FW, SO)                                                                                  \
   LAUNCH_TIMED(c, "pass_g",                                                                                          \
-               hipLaunchKernelGGL((pass_g<NEQ, T, VISC, HEAT, FW, NS, SO>), dim3(gr), dim3(kBlock), 0, c->stream,     \
+               hipLaunchKernelGGL((pass_g<NEQ, T, VISC, HEAT, FW, NS, SO>), dim3(SO ? grs : gr),                      \
+                                  dim3(SO ? gbs : kBlock), 0, c->stream,                                             \
                                   c->p, c->nc, U, c->wprim, c->link, c->geo, c->rvec, c->vol, face, gv, dg, dth,    \
                                   c->health, (int)std::min<int64_t>(c->step, INT32_MAX)))
 #define PG2(NEQ, VISC, HEAT, FW)                                                                                     \
   do {                                                                                                               \
-    if constexpr (sizeof(T) == 4 && NS == 4) {                                                                       \
+    if constexpr (NS == 4) {                                                                                         \
       if (sout) {                                                                                                    \
         PG3(NEQ, VISC, HEAT, FW, true);                                                                              \
         break;                                                                                                       \

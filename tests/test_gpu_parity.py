@@ -252,13 +252,15 @@ def test_tuning_options_do_not_change_results(gpu, precision, name):
         assert np.array_equal(o, out[0])
 
 
+@pytest.mark.parametrize("precision", [_lib.FP64, _lib.MIXED])
 @pytest.mark.parametrize("n", [10, 17])
-def test_pass_g_face_records_through_lds_are_bitwise(gpu, n):
-    """stage_g: mixed pass G writes the face records of its workgroup from LDS
-    in 16-byte words; face states, gradients and limiter equal the direct
-    stores bitwise (6000 and 29478 cells: the last workgroup is partial)."""
+def test_pass_g_face_records_through_lds_are_bitwise(gpu, n, precision):
+    """stage_g: pass G writes the face records and gradient rows of its
+    workgroup (256 cells, FP64 128) from LDS in 16-byte words; face states,
+    gradients and limiter equal the direct stores bitwise (6000 and 29478
+    cells: the last workgroup is partial)."""
     c = cases.shock_bubble(n=n, nbub=3)
-    p = dict(c.params, precision=_lib.MIXED)
+    p = dict(c.params, precision=precision)
     out = []
     for sg in (0, 1):
         with GpuSolver(p, c.mesh, c.U) as g:
