@@ -236,11 +236,11 @@ def main():
     ap.add_argument("--order", default="morton", choices=["rcm", "morton", "none"], help="cell order of the mesh")
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--stage-f", type=int, default=1, choices=[0, 1],
-                    help="pass F stages the workgroup's face states in LDS (mixed precision); same results")
+                    help="pass F stages the workgroup's face states in LDS (256 cells, FP64 128); same results")
     ap.add_argument("--fuse", type=int, default=1, choices=[0, 1],
                     help="RK update inside pass F (1) or residual + separate update kernel (0); same results")
     ap.add_argument("--stage-g", type=int, default=1, choices=[0, 1],
-                    help="pass G writes its face records through LDS (mixed precision); same results")
+                    help="pass G writes its face records through LDS (256 cells, FP64 128); same results")
     ap.add_argument("--copy-bytes", type=int, default=1 << 30,
                     help="size of the plain device copy measured as the HBM reference (0: skip)")
     args = ap.parse_args()

@@ -269,7 +269,7 @@ def test_pass_g_face_records_through_lds_are_bitwise(gpu, n, precision):
             g.set_debug(True)
             g.compute_face_values()
             out.append([g.download(k) for k in ("face", "grad", "theta")])
-    assert c.mesh.n_cells % 256 != 0
+    assert c.mesh.n_cells % 256 != 0 and c.mesh.n_cells % 128 != 0
     for a, b in zip(out[0], out[1]):
         assert np.array_equal(a, b, equal_nan=True)
 
