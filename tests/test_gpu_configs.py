@@ -10,6 +10,9 @@ step count wherever the OpenMP CPU oracle affords it, through the C ABI.
   shock-bubble  40^3 x 6 = 384,000 tets, 64 bubbles, 50 steps, in the
                 Morton cell order the benchmark times               (stated: 200^3)
 
+TGV, bubble and shock-bubble at their full stated sizes are in
+test_gpu_full_size.py (opt-in: the oracle takes minutes per case there).
+
 The bubble collapse is cut at 50 steps on the 48^3 mesh: at step 54 the
 reference scheme (the oracle, and the GPU path bitwise with it) produces a
 negative liquid partial density in a cell where the liquid converges on the
