@@ -73,6 +73,39 @@ def kernel_bytes(name, B, neq, mixed=True):
     }.get(name)
 
 
+# Entries of the counter summary for the kernels a timed run launches: the
+# mixed fused stage writes primitive records (pass_f_w); the FP64 entries
+# carry the fp64_ prefix and its last stage runs the plain relaxation.
+TRAFFIC_NAMES = {
+    "mixed_mlp": {"pass_g": "pass_g", "pass_f_rhs": "pass_f_rhs", "pass_f": "pass_f_w",
+                  "pass_f_relax": "pass_f_relax", "rk_update": "rk_update"},
+    "fp64": {"pass_g": "fp64_pass_g", "pass_f_rhs": "fp64_pass_f_rhs", "pass_f": "fp64_pass_f",
+             "pass_f_relax": "fp64_pass_f_relax_plain", "rk_update": "fp64_rk_update"},
+}
+
+
+def traffic_rate(kt, seconds, cells, leg):
+    """Bytes per second a timed run moved between L2 and the memory fabric:
+    the summary's counter bytes per launch of each kernel (FETCH_SIZE x 2 +
+    WRITE_SIZE) times its launches in the timed steps, over the wall time.
+    Gather over-fetch and Infinity Cache hits are included, so this bounds
+    the HBM traffic from above; it is the figure to set against the copy
+    rate.  None without a summary of this size; kernels without an entry
+    (the small reductions and list kernels) are left out."""
+    names = TRAFFIC_NAMES.get(leg)
+    if names is None:
+        return None
+    total = 0.0
+    for k, (n, _) in kt.items():
+        if k not in names:
+            continue
+        b = measured_traffic(names[k], cells)
+        if b is None:
+            return None
+        total += b * n
+    return total / seconds
+
+
 def measured_traffic(kernel, cells):
     """HBM bytes per launch of `kernel` (read + written) from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated
@@ -327,10 +360,17 @@ def main():
                           "frac": fam_bytes / fam_s / HBM_PEAK if fam_s else None,
                           "bytes_per_cell": {k: kernel_bytes(k, B, 10) for k in fam}},
         "whole_step_hbm_frac": whole / HBM_PEAK,
+        "whole_step_traffic": None,
         "stream_copy": copy,
     }
     if head == "mixed_mlp":
         out["surrogate_training"] = {k: v for k, v in mixed["info"].items() if k != "relaxation"}
+    tr = traffic_rate(kt, mixed["el"], C, head)
+    if tr is not None:
+        out["whole_step_traffic"] = {"GBps": tr / 1e9, "frac_of_peak": tr / HBM_PEAK,
+                                     "frac_of_copy": tr / (copy["GBps"] * 1e9) if copy and copy.get("GBps") else None,
+                                     "source": "profiles/shock_bubble_48M_traffic.json bytes per launch x launches / "
+                                               "wall time; includes over-fetch and Infinity Cache hits"}
     out["relaxation"] = {k: r["info"].get("relaxation") for k, r in res.items()}
     for other in ("mixed_cold", "mixed"):
         if other in res and other != head:
@@ -343,6 +383,9 @@ def main():
         out["fp64"] = {"value": f64["rate"], "ms_per_step": f64["el"] / args.steps * 1e3,
                        "whole_step_hbm_frac": (Bd["total"] * C * s * args.steps + relax_bytes(10) * C * args.steps)
                        / f64["el"] / HBM_PEAK}
+        tr64 = traffic_rate(f64["kt"], f64["el"], C, "fp64")
+        if tr64 is not None:
+            out["fp64"]["whole_step_traffic_frac_of_peak"] = tr64 / HBM_PEAK
         out["speedup_mixed_vs_fp64"] = mixed["rate"] / f64["rate"]
         out["mixed_vs_fp64_gpu_rel_l1"] = rel_errors(mixed["U"], f64["U"])
     del res, case
