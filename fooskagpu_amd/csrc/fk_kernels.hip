@@ -149,6 +149,27 @@ constexpr int gout_stride() {
   return 4 * w;
 }
 
+// Workgroup of pass G: kBlock cells, or with the LDS store path (SOUT) the
+// cells whose face records fill 30-40 KB, so that four workgroups share a
+// CU's LDS: 4 slots 256 cells (FP64 128, its occupancy of 2 waves per SIMD),
+// 6 slots (FP32) 128.
+template <typename T, int NS, bool SOUT>
+constexpr int pass_g_block() {
+  if (!SOUT) return kBlock;
+  if (NS == 4) return sizeof(T) == 8 ? 128 : kBlock;
+  return 128;
+}
+
+// Record layouts that take the LDS store path: each cell's face records are
+// whole 16-byte words (not single-phase FP32 with 6 slots, 120 bytes), and
+// not FP64 with 6 slots, whose 480-byte records leave 64-cell workgroups
+// (pass G 3.29 against 3.21 ms with direct stores on 200^3 hexahedra; FP32
+// with 6 slots: 1.53 against 2.05 ms).
+template <typename T, int NS, int NEQ>
+constexpr bool pass_g_sout_ok() {
+  return NS * NEQ * (int)sizeof(T) % 16 == 0 && (NS == 4 || sizeof(T) == 4);
+}
+
 template <int NEQ, typename T, bool VISC, bool HEAT, bool FROMW, int NS, bool SOUT = false>
 __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
     pass_g(fk_params P, int64_t nc, const double *__restrict__ U, const float *__restrict__ Wp,
@@ -156,11 +177,8 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
            const double *__restrict__ vol, T *__restrict__ face,
            T *__restrict__ gvel, double *__restrict__ dgrad, double *__restrict__ dtheta,
            unsigned long long *__restrict__ health, int step) {
-  static_assert(!SOUT || NS * NEQ * sizeof(T) % 16 == 0, "face records staged as 16-byte words");
-  // SOUT workgroup: 256 cells, FP64 128 (its 320-byte records fill 40 KB,
-  // so that four workgroups, the FP64 occupancy of 2 waves per SIMD, fit
-  // in a CU's LDS)
-  constexpr int GB = (SOUT && sizeof(T) == 8) ? 128 : kBlock;
+  static_assert(!SOUT || pass_g_sout_ok<T, NS, NEQ>(), "face records staged as 16-byte words");
+  constexpr int GB = pass_g_block<T, NS, SOUT>();
   // T per cell in the LDS copy of the face records (FP64: unpadded), and floats
   constexpr int SPT = sizeof(T) == 4 ? gout_stride<NS * NEQ>() : NS * NEQ;
   constexpr int SP = SPT * (int)sizeof(T) / 4;
@@ -1092,10 +1110,9 @@ int launch_pass_g_t(fk_ctx *c) {
   T *face = (T *)c->face;
   T *gv = (T *)c->gvel;
   const bool fromw = sizeof(T) == 4 && c->prims_w && c->w_valid;
-  // face records through LDS (4 slots per cell): workgroups of 256 cells,
-  // FP64 128
-  const bool sout = NS == 4 && c->stage_g;
-  constexpr int gbs = sizeof(T) == 8 ? 128 : kBlock;
+  // face records through LDS (workgroup sizes: pass_g_block)
+  const bool sout = c->stage_g;
+  constexpr int gbs = pass_g_block<T, NS, true>();
   const unsigned grs = (unsigned)(((c->nc + gbs - 1) / gbs + kXcd - 1) / kXcd * kXcd);
 #define PG3(NEQ, VISC, HEAT, FW, SO)                                                                                  \
   LAUNCH_TIMED(c, "pass_g",                                                                                          \
@@ -1105,7 +1122,7 @@ int launch_pass_g_t(fk_ctx *c) {
                                   c->health, (int)std::min<int64_t>(c->step, INT32_MAX)))
 #define PG2(NEQ, VISC, HEAT, FW)                                                                                     \
   do {                                                                                                               \
-    if constexpr (NS == 4) {                                                                                         \
+    if constexpr (pass_g_sout_ok<T, NS, NEQ>()) {                                                                    \
       if (sout) {                                                                                                    \
         PG3(NEQ, VISC, HEAT, FW, true);                                                                              \
         break;                                                                                                       \

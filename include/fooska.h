@@ -214,9 +214,9 @@ int fk_set_debug(fk_ctx *ctx, int on);
  *  "stage_f" (default 1): meshes of tetrahedra; pass F copies the face
  *     states and velocity gradients of its workgroup's cells (256, FP64 128)
  *     to LDS and reads in-workgroup mirror states from there.
- *  "stage_g" (default 1): meshes of tetrahedra; pass G writes its
- *     workgroup's face records and velocity gradients (256 cells, FP64 128)
- *     from LDS in 16-byte words.
+ *  "stage_g" (default 1): pass G writes its workgroup's face records and
+ *     velocity gradients from LDS in 16-byte words (4 slots per cell: 256
+ *     cells, FP64 128; 6 slots: two-phase FP32, 128 cells).
  *  "prims_w" (default 1): mixed precision only; the kernels that produce a
  *     new state (fused pass F, relaxation) also write its FP32 primitive
  *     variables, which the next pass G gathers instead of converting the FP64

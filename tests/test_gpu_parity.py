@@ -274,6 +274,29 @@ def test_pass_g_face_records_through_lds_are_bitwise(gpu, n, precision):
         assert np.array_equal(a, b, equal_nan=True)
 
 
+@pytest.mark.parametrize("precision", [_lib.FP64, _lib.MIXED])
+@pytest.mark.parametrize("model", [_lib.MODEL_TWO_PHASE, _lib.MODEL_SINGLE])
+def test_pass_g_face_records_through_lds_six_slots(gpu, model, precision):
+    """stage_g on a mesh of every element type (6 slots per cell): the LDS
+    store path (two-phase FP32, workgroups of 128 cells) equals the direct
+    stores bitwise, including the empty slots of cells with fewer than six
+    faces and the partial last workgroup; the other layouts keep the direct
+    stores either way."""
+    c = mixed_elements(model)
+    p = dict(c.params, precision=precision)
+    out = []
+    for sg in (0, 1):
+        with GpuSolver(p, c.mesh, c.U) as g:
+            g.set_tuning("stage_g", sg)
+            g.run_steps(2)
+            g.set_debug(True)
+            g.compute_face_values()
+            out.append([g.download(k) for k in ("face", "grad", "theta")] + [g.download_state()])
+    assert c.mesh.n_cells % 64 != 0
+    for a, b in zip(out[0], out[1]):
+        assert np.array_equal(a, b, equal_nan=True)
+
+
 @pytest.mark.parametrize("name,mode", [("cavitation", _lib.RELAX_PLAIN), ("cavitation", _lib.RELAX_COLD),
                                        ("shock_bubble", _lib.RELAX_COLD)])
 def test_primitive_records_of_relaxation(gpu, name, mode):
