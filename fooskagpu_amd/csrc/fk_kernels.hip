@@ -418,13 +418,28 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
 // Workgroup size: 256 cells; 128 for FP64 with staging, whose face records
 // and gradients take 52 KB of LDS per 128 cells (104 KB at 256 would leave
 // one workgroup per CU).
-template <typename T, bool STAGE>
+// Workgroup of pass F: kBlock cells, or with LDS staging (STAGE) 256 cells
+// (49 KB) with 4 slots (FP64 128, 52 KB), 128 cells (37 KB) with 6 slots.
+template <typename T, bool STAGE, int NS = 4>
 constexpr int pass_f_block() {
-  return (STAGE && sizeof(T) == 8) ? 128 : kBlock;
+  if (!STAGE) return kBlock;
+  if (NS == 4) return sizeof(T) == 8 ? 128 : kBlock;
+  return 128;
+}
+
+// Layouts that pass F stages: 4 slots; 6 slots for two-phase FP32 records
+// (whole 16-byte words per workgroup row, 37 KB per 128 cells).  On 200^3
+// hexahedra (8M cells) the fused stage went from 2.61 to 1.80 ms and the dt
+// stage from 2.46 to 1.75 ms; the last stage with the per-cell relaxation
+// slowed down there (2.74 to 3.01 ms, its registers at the lower occupancy
+// of the larger LDS share), so with 6 slots it stays unstaged.
+template <typename T, int NS, int NEQ>
+constexpr bool pass_f_stage_ok() {
+  return NS == 4 || (sizeof(T) == 4 && NEQ == 10);
 }
 
 template <int NEQ, typename T, bool VISC, bool HEAT, int MODE, bool STAGE, int NS, bool WOUT = false, int RLX = 0>
-__global__ void __launch_bounds__((pass_f_block<T, STAGE>()), ((MODE == 1 || sizeof(T) == 8) ? 2 : 3)) pass_f(fk_params P, int64_t nc, const double *__restrict__ U,
+__global__ void __launch_bounds__((pass_f_block<T, STAGE, NS>()), ((MODE == 1 || sizeof(T) == 8) ? 2 : 3)) pass_f(fk_params P, int64_t nc, const double *__restrict__ U,
                                                  const double *__restrict__ U0, double *__restrict__ Unew,
                                                  const int32_t *__restrict__ link,
                                                  const void *__restrict__ geo, const T *__restrict__ wls,
@@ -438,7 +453,7 @@ __global__ void __launch_bounds__((pass_f_block<T, STAGE>()), ((MODE == 1 || siz
   static_assert(!RLX || (MODE == 0 && NEQ == 10), "fused relaxation: two-phase RK stage only");
   constexpr int KR = RLX == 2 ? 3 : 2;  // counts per wave in rpart
   constexpr int NGS = ngrad<NEQ, HEAT>();
-  constexpr int BS = pass_f_block<T, STAGE>();
+  constexpr int BS = pass_f_block<T, STAGE, NS>();
   const int64_t c0 = xcd_block() * BS;
   const int64_t c = c0 + threadIdx.x;
   extern __shared__ __attribute__((aligned(16))) unsigned char lds_raw[];
@@ -1169,10 +1184,11 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
   // fused relaxation (last stage): records of the relaxed state, mixed precision
   float *Wr = (sizeof(T) == 4 && c->prims_w) ? c->wprim : nullptr;
   unsigned long long *cnt = (unsigned long long *)c->lcount;
-  // LDS staging of face states and gradients (tetrahedral meshes): mixed
-  // precision 256 cells, 49 KB per workgroup; FP64 128 cells, 52 KB
-  const bool stage = c->stage_f && NS == 4;
-  constexpr int bs = pass_f_block<T, true>();
+  // LDS staging of face states and gradients (pass_f_stage_ok,
+  // pass_f_block); with 6 slots not for the last stage's relaxation pass
+  const bool stage = c->stage_f && (NS == 4 || relax == 0) &&
+                     (c->neq == 10 ? pass_f_stage_ok<T, NS, 10>() : pass_f_stage_ok<T, NS, 5>());
+  constexpr int bs = pass_f_block<T, true, NS>();
   const unsigned gs = (unsigned)(((c->nc + bs - 1) / bs + kXcd - 1) / kXcd * kXcd);
   const size_t lds = stage ? (size_t)bs * (NS * c->neq + 12) * sizeof(T) : 0;
   // waves of this launch (per-wave relaxation counts of the fused last stage)
@@ -1208,7 +1224,7 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
     if constexpr (MODE == 0 && sizeof(T) == 4) {                                                                       \
       if (Wout) {                                                                                                      \
         if (stage)                                                                                                     \
-          LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, (NS == 4), NS, true>), dim3(gs),             \
+          LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, (pass_f_stage_ok<T, NS, NEQ>()), NS, true>), dim3(gs),             \
                                                  dim3(bs), lds, c->stream, PF_ARGS(Wout), nullptr, nullptr,             \
                                                  nullptr));                                                             \
         else                                                                                                           \
@@ -1219,7 +1235,7 @@ int launch_pass_f_t(fk_ctx *c, int irk, double dt, const double *dtp, double *Un
       }                                                                                                                \
     }                                                                                                                  \
     if (stage)                                                                                                         \
-      LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, (NS == 4), NS>), dim3(gs), dim3(bs), lds,         \
+      LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, (pass_f_stage_ok<T, NS, NEQ>()), NS>), dim3(gs), dim3(bs), lds,         \
                                              c->stream, PF_ARGS(Wout), nullptr, nullptr, nullptr));                     \
     else                                                                                                               \
       LAUNCH_TIMED(c, nm, hipLaunchKernelGGL((pass_f<NEQ, T, VISC, HEAT, MODE, false, NS>), dim3(gr), dim3(kBlock), 0,      \

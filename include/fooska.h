@@ -211,9 +211,10 @@ int fk_set_debug(fk_ctx *ctx, int on);
  *     reduced, and a streaming kernel applies the update); at 48M cells,
  *     mixed: 7.5 ms against 6.7 + 2.5 ms.  0 = always residual + update
  *     kernel.
- *  "stage_f" (default 1): meshes of tetrahedra; pass F copies the face
- *     states and velocity gradients of its workgroup's cells (256, FP64 128)
- *     to LDS and reads in-workgroup mirror states from there.
+ *  "stage_f" (default 1): pass F copies the face states and velocity
+ *     gradients of its workgroup's cells to LDS and reads in-workgroup mirror
+ *     states from there (4 slots per cell: 256 cells, FP64 128; 6 slots:
+ *     two-phase FP32, 128 cells, not in the last stage's relaxation pass).
  *  "stage_g" (default 1): pass G writes its workgroup's face records and
  *     velocity gradients from LDS in 16-byte words (4 slots per cell: 256
  *     cells, FP64 128; 6 slots: two-phase FP32, 128 cells).

@@ -227,7 +227,7 @@ def test_cons_calc_matches_oracle(gpu):
     assert np.allclose(a, b, rtol=1e-12, atol=0)
 
 
-@pytest.mark.parametrize("name", ["shock_bubble", "tgv", "sod"])
+@pytest.mark.parametrize("name", ["shock_bubble", "tgv", "sod", "mixed_elements_tp", "mixed_elements_sp"])
 @pytest.mark.parametrize("precision", [_lib.FP64, _lib.MIXED])
 def test_tuning_options_do_not_change_results(gpu, precision, name):
     """fuse_update (RK update inside pass F or residual + separate update),
@@ -235,7 +235,8 @@ def test_tuning_options_do_not_change_results(gpu, precision, name):
     FP32 primitive records written by the producers of U) and stage_g (pass G
     writes its face records through LDS) are implementation choices: bitwise
     equal states after several steps (two-phase with relaxation,
-    single-phase viscous, single-phase inviscid)."""
+    single-phase viscous, single-phase inviscid, and both models on a mesh
+    of every element type, 6 slots per cell)."""
     c = SMALL[name]()
     p = dict(c.params, precision=precision)
     out = []
