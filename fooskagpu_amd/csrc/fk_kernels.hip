@@ -415,11 +415,10 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
 // ([n_chem, iterations], and Tsat iterations for 2); Wout gets the primitive
 // records of the cells relaxed here.
 //
-// Workgroup size: 256 cells; 128 for FP64 with staging, whose face records
-// and gradients take 52 KB of LDS per 128 cells (104 KB at 256 would leave
-// one workgroup per CU).
-// Workgroup of pass F: kBlock cells, or with LDS staging (STAGE) 256 cells
-// (49 KB) with 4 slots (FP64 128, 52 KB), 128 cells (37 KB) with 6 slots.
+// Workgroup size (pass_f_block): 256 cells; with staging, FP64 128 cells,
+// whose face records and gradients take 52 KB of LDS per 128 cells (104 KB
+// at 256 would leave one workgroup per CU), and 128 cells (37 KB) with 6
+// slots.
 template <typename T, bool STAGE, int NS = 4>
 constexpr int pass_f_block() {
   if (!STAGE) return kBlock;
