@@ -143,6 +143,8 @@ __device__ __forceinline__ int link_of(T w) {
 // floats measured the same at 48M cells (bank conflicts of the row writes
 // do not show), and 40 (40 KB per workgroup) lets four workgroups share a
 // CU's LDS.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
 template <int N>
 constexpr int gout_stride() {
   constexpr int w = (N + 3) / 4;
@@ -340,7 +342,12 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
   if constexpr (SOUT) {
     // the block's face records are one contiguous range: written from LDS in
     // 16-byte words, each wave store covering 1 KB contiguously (a thread's
-    // own 160-byte record otherwise goes out as strided partial lines)
+    // own 160-byte record otherwise goes out as strided partial lines).
+    // Non-temporal: the records are read once, by the next pass, and whole
+    // lines written past L2 leave it to the neighbour records this pass
+    // gathers (mixed pass G 5.27 against 5.49 ms, FP64 10.83 against 11.12 ms
+    // at 48M cells; with partial lines, before this path, such stores had
+    // measured 3.5 times slower).
     __syncthreads();
     constexpr int WPC = NS * NEQ * (int)sizeof(T) / 16;  // 16-byte words per cell
     const int64_t ncell = (nc - c0) < GB ? (nc - c0) : GB;
@@ -348,7 +355,8 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
     uint4 *dst = reinterpret_cast<uint4 *>(face + c0 * NS * NEQ);
     for (int w = threadIdx.x; w < nw; w += GB) {
       const int cell = w / WPC, part = w - cell * WPC;
-      dst[w] = *reinterpret_cast<const uint4 *>(sOut + cell * SP + part * 4);
+      __builtin_nontemporal_store(*reinterpret_cast<const u32x4 *>(sOut + cell * SP + part * 4),
+                                  reinterpret_cast<u32x4 *>(&dst[w]));
     }
   }
   if constexpr (VISC) {
@@ -375,7 +383,7 @@ __global__ void __launch_bounds__(kBlock, (sizeof(T) == 8 ? 2 : 1))
       const int64_t ncell = (nc - c0) < GB ? (nc - c0) : GB;
       const int nf = (int)ncell * NG;
       T *dg = gvel + c0 * NG;
-      for (int f = threadIdx.x; f < nf; f += GB) dg[f] = reinterpret_cast<const T *>(sOut)[f];
+      for (int f = threadIdx.x; f < nf; f += GB) __builtin_nontemporal_store(reinterpret_cast<const T *>(sOut)[f], &dg[f]);
     }
   }
   if (SOUT && !live) return;
